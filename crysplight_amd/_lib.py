@@ -1,0 +1,160 @@
+"""ctypes binding of libcrysplight_hip.so (include/crysplight.h).
+
+The product path has no CPU fallback: if the shared object is missing or no HIP
+device is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+from . import params as P
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libcrysplight_hip.so"
+HEADER = HERE.parent / "include" / "crysplight.h"
+
+_lib = None
+
+_vp = C.c_void_p
+_i32, _i64, _u64, _f32 = C.c_int32, C.c_int64, C.c_uint64, C.c_float
+_pbox, _pop, _pro = C.POINTER(P.Box), C.POINTER(P.OpticalParams), C.POINTER(P.Readout)
+_ptb, _pem = C.POINTER(P.TimeBinning), C.POINTER(P.Emission)
+
+SIGNATURES = {
+    "crysp_create": (_vp, [C.c_int]),
+    "crysp_destroy": (None, [_vp]),
+    "crysp_last_error": (C.c_char_p, []),
+    "crysp_abi_version": (C.c_int, []),
+    "crysp_transport_records": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _vp, _i32,
+                                          _i64, _u64, _u64, _i32, _pem, _vp, _vp, _vp, _vp]),
+    "crysp_transport_accumulate": (C.c_int, [_vp, _pbox, _pop, _pro, _ptb, _vp, _vp, _vp, _vp,
+                                             _i32, _i64, _u64, _u64, _i32, _pem, _vp, _vp, _vp]),
+    "crysp_run_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
+                                   _u64, _i32, _pem, _vp, _vp, _vp, _vp]),
+    "crysp_set_schedule": (C.c_int, [_vp, _i32, _i32]),
+    "crysp_get_schedule": (C.c_int, [_vp, _vp, _vp]),
+    "crysp_autotune_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
+                                        _i32, _pem, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
+    "crysp_debug_draws": (C.c_int, [_vp, _u64, _u64, _i64, _i32, _vp]),
+    "crysp_debug_math": (C.c_int, [_vp, _i32, _vp, _i64, _vp]),
+    "crysp_set_timing": (C.c_int, [_vp, _i32]),
+    "crysp_kernel_time": (C.c_int, [_vp, _vp, _vp, _i32]),
+    "crysp_set_op_counting": (C.c_int, [_vp, _i32]),
+    "crysp_get_op_counts": (C.c_int, [_vp, _vp, _i32]),
+}
+
+
+class CryspError(RuntimeError):
+    pass
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/crysplight.h."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(crysp_[a-z0-9_]+)\s*\(", text)))
+
+
+def load() -> C.CDLL:
+    """Load the shared object (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise CryspError(f"{LIB_PATH} is missing: build it with `python -m crysplight_amd.build`"
+                         " (there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().crysp_last_error().decode(errors="replace")
+        raise CryspError(f"{what} failed ({rc}): {msg}")
+
+
+class Context:
+    """One library context (HIP stream, device buffers, graphs, schedule) on a device.
+    Replaces the ArrayT backend selector (src/kernel.jl:319)."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = lib.crysp_create(int(device))
+        if not h:
+            raise CryspError("crysp_create failed: " + lib.crysp_last_error().decode())
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().crysp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # schedule / timing -------------------------------------------------------
+    def set_schedule(self, wg: int, ppt: int):
+        check(load().crysp_set_schedule(self._h, wg, ppt), "crysp_set_schedule")
+
+    def get_schedule(self) -> tuple[int, int]:
+        a, b = C.c_int32(), C.c_int32()
+        check(load().crysp_get_schedule(self._h, C.byref(a), C.byref(b)), "crysp_get_schedule")
+        return a.value, b.value
+
+    def set_timing(self, on: bool):
+        check(load().crysp_set_timing(self._h, int(on)), "crysp_set_timing")
+
+    def kernel_time(self, reset: bool = False) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int64()
+        check(load().crysp_kernel_time(self._h, C.byref(ms), C.byref(n), int(reset)),
+              "crysp_kernel_time")
+        return ms.value, n.value
+
+    def set_op_counting(self, on: bool):
+        check(load().crysp_set_op_counting(self._h, int(on)), "crysp_set_op_counting")
+
+    def op_counts(self, reset: bool = False):
+        import numpy as np
+        out = np.zeros(10, np.int64)
+        check(load().crysp_get_op_counts(self._h, out.ctypes.data, int(reset)),
+              "crysp_get_op_counts")
+        return out
+
+
+_default: dict[int, Context] = {}
+
+
+def default_context(device: int = 0) -> Context:
+    ctx = _default.get(device)
+    if ctx is None:
+        ctx = _default[device] = Context(device)
+    return ctx
+
+
+def ptr(a) -> int | None:
+    """Address of a numpy array or a torch tensor (device or host); None for None."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def env_flag(name: str) -> bool:
+    return os.environ.get(name, "") not in ("", "0")

@@ -1,0 +1,402 @@
+// crysp_device.h -- per-lane physics of the optical-photon transport on gfx950.
+//
+// One lane owns one photon at a time.  Everything here is IEEE binary32 arithmetic
+// compiled with -ffp-contract=off and correctly rounded division / sqrt, so each
+// operation gives the same bits as the Julia Float32 code it follows.  The
+// transcendentals (log, sin, cos) are evaluated in binary64 and rounded once to
+// binary32; Julia's Float32 versions are computed the same way and are correctly
+// rounded in practice.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crysp {
+
+enum : int { ST_FLYING = 0, ST_DETECTED = 1, ST_ABS_BULK = 2, ST_ABS_WALL = 3,
+             ST_ABS_SIPM = 4, ST_CAP = 5, ST_ABS_SUR = 6 };
+// face codes, src/geometry.jl:4-9
+enum : int { F_XP = 1, F_XM = 2, F_YP = 3, F_YM = 4, F_BACK = 5, F_FRONT = 6 };
+
+// flag bits of KParams::flags
+enum : uint32_t { K_ABS = 1u, K_RAY = 2u, K_WRAP_SPEC = 4u, K_AIR_GAP = 8u, K_DISC = 16u,
+                  K_SUR_SPEC = 32u, K_TWO_SIDED = 64u };
+
+constexpr float TWO_PI_F = 6.2831855f;        // 2f0 * Float32(pi): exact product
+constexpr float C_MM_NS = 299.792458f;        // src/transport.jl:4
+
+// Everything a launch needs that is the same for all lanes (kernarg -> SGPRs).
+struct KParams {
+    float L0, L1, L2;
+    float n_crystal, n_coupling, abs_len, ray_len, wrap_R, sigma_alpha, pde;
+    float inv_v;                              // n_crystal / c  [ns/mm]
+    float cx, cy, r2, sur_R;
+    float pitch, bin_ns;
+    float tau_d, tau_r;
+    int32_t nx, ny, nbins, maxb;
+    uint32_t flags;
+    uint32_t key0, key1;                      // Philox key = (seed lo, seed hi)
+    int32_t npos;
+    uint64_t pid0;
+    int64_t n_photons;
+    int64_t ndep;
+    int32_t chunk;                            // photons per work-queue grab
+    int32_t hist_words;                       // nx*ny*(nbins+1)
+};
+
+// ------------------------------------------------------------------ Philox ----
+// Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
+__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                         uint32_t k0, uint32_t k1, uint32_t &o0,
+                                         uint32_t &o1, uint32_t &o2, uint32_t &o3)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+// Per-photon stream: counter (pid lo, pid hi, block, 0); four words per block, used
+// in order (src/philox.jl:34-57).  The words shift down as they are consumed.
+struct Rng {
+    uint32_t lo, hi, blk, left;
+    uint32_t w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ void rng_init(Rng &r, uint64_t pid)
+{
+    r.lo = (uint32_t)pid; r.hi = (uint32_t)(pid >> 32); r.blk = 0u; r.left = 0u;
+    r.w0 = r.w1 = r.w2 = r.w3 = 0u;
+}
+
+template <bool COUNT>
+struct OpCount {
+    // n_photon, n_draw, n_block, n_step, n_fresnel, n_surface_try, n_lambert,
+    // n_rayleigh_try, n_detect, n_nscat
+    uint32_t v[10];
+    __device__ __forceinline__ void add(int k, uint32_t d = 1u) { if (COUNT) v[k] += d; }
+};
+
+template <bool COUNT>
+__device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<COUNT> &oc)
+{
+    if (r.left == 0u) {
+        philox10(r.lo, r.hi, r.blk, 0u, P.key0, P.key1, r.w0, r.w1, r.w2, r.w3);
+        r.blk += 1u;
+        r.left = 4u;
+        oc.add(2);
+    }
+    oc.add(1);
+    const uint32_t x = r.w0;
+    r.w0 = r.w1; r.w1 = r.w2; r.w2 = r.w3;
+    r.left -= 1u;
+    return x;
+}
+
+// ((x >>> 8) + 1) * 2^-24 in (0, 1] (src/philox.jl:62): every step exact.
+__device__ __forceinline__ float unit_of(uint32_t x)
+{
+    return ((float)(x >> 8) + 1.0f) * 5.9604645e-08f;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> &oc)
+{
+    return unit_of(rng_next<COUNT>(r, P, oc));
+}
+
+// ----------------------------------------------------------- transcendentals ----
+__device__ __forceinline__ float f_log(float x) { return (float)::log((double)x); }
+__device__ __forceinline__ void f_sincos(float x, float &s, float &c)
+{
+    double sd, cd;
+    ::sincos((double)x, &sd, &cd);
+    s = (float)sd;
+    c = (float)cd;
+}
+__device__ __forceinline__ float f_cos(float x) { return (float)::cos((double)x); }
+
+// Julia Base semantics used by the reference
+__device__ __forceinline__ float jmax0(float v) { return (v != v) ? v : (v > 0.0f ? v : 0.0f); }
+__device__ __forceinline__ float jclamp(float x, float lo, float hi)
+{
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+
+// ----------------------------------------------------------------- optics ----
+// Unpolarized Fresnel with TIR (src/optics.jl:10-19).  Returns true on TIR.
+__device__ __forceinline__ bool fresnel(float n1, float n2, float cosi, float &R)
+{
+    const float q = n1 / n2;
+    const float sin2t = (q * q) * (1.0f - cosi * cosi);
+    if (sin2t >= 1.0f) { R = 1.0f; return true; }
+    const float cost = sqrtf(1.0f - sin2t);
+    const float a = (n1 * cosi - n2 * cost) / (n1 * cosi + n2 * cost);
+    const float b = (n1 * cost - n2 * cosi) / (n1 * cost + n2 * cosi);
+    R = 0.5f * (a * a + b * b);
+    return false;
+}
+
+__device__ __forceinline__ int face_axis(int f) { return f <= 2 ? 1 : (f <= 4 ? 2 : 3); }
+
+__device__ __forceinline__ void specular(float &ux, float &uy, float &uz, int face)
+{
+    const int a = face_axis(face);
+    if (a == 1) ux = -ux; else if (a == 2) uy = -uy; else uz = -uz;
+}
+
+// Lambertian re-injection about the inward normal (src/optics.jl:118-138).
+template <bool COUNT>
+__device__ __forceinline__ void lambertian(Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                           int face, float &ux, float &uy, float &uz)
+{
+    oc.add(6);
+    const float ct = sqrtf(randu<COUNT>(r, P, oc));
+    const float st = sqrtf(jmax0(1.0f - ct * ct));
+    const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    const float a = st * cp, b = st * sp;
+    switch (face) {
+    case F_XP: ux = -ct; uy = a; uz = b; break;
+    case F_XM: ux = ct; uy = a; uz = b; break;
+    case F_YP: ux = a; uy = -ct; uz = b; break;
+    case F_YM: ux = a; uy = ct; uz = b; break;
+    case F_BACK: ux = a; uy = b; uz = -ct; break;
+    default: ux = a; uy = b; uz = ct; break;
+    }
+}
+
+// Rayleigh (1 + cos^2) scatter about the current direction (src/optics.jl:87-112).
+template <bool COUNT>
+__device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                         float &ux, float &uy, float &uz)
+{
+    float mu;
+    for (;;) {
+        oc.add(7);
+        mu = 1.0f - 2.0f * randu<COUNT>(r, P, oc);
+        const float u2 = randu<COUNT>(r, P, oc);
+        if (2.0f * u2 <= 1.0f + mu * mu) break;
+    }
+    const float st = sqrtf(jmax0(1.0f - mu * mu));
+    const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    const float cb = st * cp, sb = st * sp;
+    float t1x, t1y, t1z, inv;
+    if (fabsf(uz) < 0.9f) {
+        inv = 1.0f / sqrtf(ux * ux + uy * uy);
+        t1x = -uy * inv; t1y = ux * inv; t1z = 0.0f;
+    } else {
+        inv = 1.0f / sqrtf(uy * uy + uz * uz);
+        t1x = 0.0f; t1y = -uz * inv; t1z = uy * inv;
+    }
+    const float t2x = uy * t1z - uz * t1y;
+    const float t2y = uz * t1x - ux * t1z;
+    const float t2z = ux * t1y - uy * t1x;
+    const float vx = mu * ux + cb * t1x + sb * t2x;
+    const float vy = mu * uy + cb * t1y + sb * t2y;
+    const float vz = mu * uz + cb * t1z + sb * t2z;
+    inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
+    ux = vx * inv; uy = vy * inv; uz = vz * inv;
+}
+
+// UNIFIED specular-lobe interaction at the rough crystal-air interface
+// (src/optics.jl:49-81): up to 20 micro-facet tries, fallback ideal specular.
+// Returns true when the photon is transmitted into the gap.
+template <bool COUNT>
+__device__ __forceinline__ bool unified_surface(Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                                float &ux, float &uy, float &uz, int face)
+{
+    const int a = face_axis(face);
+    const float sgn = (face == F_XP || face == F_YP || face == F_BACK) ? 1.0f : -1.0f;
+    for (int k = 0; k < 20; ++k) {
+        oc.add(5);
+        const float u1 = randu<COUNT>(r, P, oc);
+        const float u2 = randu<COUNT>(r, P, oc);
+        const float g = sqrtf(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
+        const float alpha = P.sigma_alpha * fabsf(g);
+        if (alpha >= 1.2f) continue;
+        float sa, ca;
+        f_sincos(alpha, sa, ca);
+        const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+        float sp, cp;
+        f_sincos(phi, sp, cp);
+        const float b = sa * cp, c = sa * sp;
+        const float nfx = a == 1 ? ca * sgn : b;
+        const float nfy = a == 2 ? ca * sgn : (a == 1 ? b : c);
+        const float nfz = a == 3 ? ca * sgn : c;
+        const float cosi = ux * nfx + uy * nfy + uz * nfz;
+        if (cosi <= 0.0f) continue;
+        float R;
+        oc.add(4);
+        const bool tir = fresnel(P.n_crystal, 1.0f, cosi, R);
+        bool refl = tir;
+        if (!tir) refl = randu<COUNT>(r, P, oc) < R;
+        if (!refl) return true;
+        const float rx = ux - 2.0f * cosi * nfx;
+        const float ry = uy - 2.0f * cosi * nfy;
+        const float rz = uz - 2.0f * cosi * nfz;
+        // inward check against the outward normal (only its face-axis component is nonzero)
+        const float rn = a == 1 ? rx * sgn : (a == 2 ? ry * sgn : rz * sgn);
+        if (rn < 0.0f) { ux = rx; uy = ry; uz = rz; return false; }
+    }
+    specular(ux, uy, uz, face);
+    return false;
+}
+
+// Emission (src/generation.jl:4-13, src/kernel.jl:175-180) + rise time.
+template <bool COUNT>
+__device__ __forceinline__ void emit(Rng &r, const KParams &P, OpCount<COUNT> &oc, float t0,
+                                     float &ux, float &uy, float &uz, float &t)
+{
+    const float u1 = randu<COUNT>(r, P, oc);
+    const float z = 1.0f - 2.0f * u1;
+    const float rr = sqrtf(jmax0(1.0f - z * z));
+    const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    ux = rr * cp; uy = rr * sp; uz = z;
+    t = t0;
+    if (P.tau_d > 0.0f) t += -P.tau_d * f_log(randu<COUNT>(r, P, oc));
+    if (P.tau_r > 0.0f) t += -P.tau_r * f_log(randu<COUNT>(r, P, oc));
+}
+
+struct Photon {
+    float x, y, z, ux, uy, uz, t;
+    int32_t bounces;
+};
+
+// One iteration of the bounce loop (src/kernel.jl:185-298).  Returns the photon's
+// status after the step (ST_FLYING to continue); `lin` gets the detected SiPM index.
+template <bool COUNT>
+__device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                    int &lin, int &nscat)
+{
+    oc.add(3);
+    // wall_hit (src/geometry.jl:19-32), ties x -> y -> z
+    const float inf = __builtin_inff();
+    const float dx = p.ux > 0.0f ? (P.L0 - p.x) / p.ux : (p.ux < 0.0f ? -p.x / p.ux : inf);
+    const float dy = p.uy > 0.0f ? (P.L1 - p.y) / p.uy : (p.uy < 0.0f ? -p.y / p.uy : inf);
+    const float dz = p.uz > 0.0f ? (P.L2 - p.z) / p.uz : (p.uz < 0.0f ? -p.z / p.uz : inf);
+    float d;
+    int face;
+    if (dx <= dy && dx <= dz) { d = dx; face = p.ux > 0.0f ? F_XP : F_XM; }
+    else if (dy <= dz)        { d = dy; face = p.uy > 0.0f ? F_YP : F_YM; }
+    else                      { d = dz; face = p.uz > 0.0f ? F_BACK : F_FRONT; }
+
+    float d_abs = inf, d_ray = inf;
+    if (P.flags & K_ABS) d_abs = -P.abs_len * f_log(randu<COUNT>(r, P, oc));
+    if (P.flags & K_RAY) d_ray = -P.ray_len * f_log(randu<COUNT>(r, P, oc));
+    if (d_abs < d && d_abs <= d_ray) return ST_ABS_BULK;
+    if (d_ray < d) {
+        p.x += d_ray * p.ux; p.y += d_ray * p.uy; p.z += d_ray * p.uz;
+        p.t += d_ray * P.inv_v;
+        rayleigh<COUNT>(r, P, oc, p.ux, p.uy, p.uz);
+        ++nscat;
+        oc.add(9);
+        ++p.bounces;
+        return p.bounces >= P.maxb ? ST_CAP : ST_FLYING;
+    }
+    // advance to the wall, snap onto it, clamp the rest (src/kernel.jl:206-218)
+    p.x = jclamp(p.x + d * p.ux, 0.0f, P.L0);
+    p.y = jclamp(p.y + d * p.uy, 0.0f, P.L1);
+    p.z = jclamp(p.z + d * p.uz, 0.0f, P.L2);
+    const int a = face_axis(face);
+    if (a == 1)      p.x = face == F_XP ? P.L0 : 0.0f;
+    else if (a == 2) p.y = face == F_YP ? P.L1 : 0.0f;
+    else             p.z = face == F_BACK ? P.L2 : 0.0f;
+    p.t += d * P.inv_v;
+    const float cosi = a == 1 ? fabsf(p.ux) : (a == 2 ? fabsf(p.uy) : fabsf(p.uz));
+
+    int result = ST_FLYING;
+    if (face == F_BACK || ((P.flags & K_TWO_SIDED) && face == F_FRONT)) {
+        // readout face: crystal -> coupling Fresnel, then PDE (src/kernel.jl:220-251)
+        float R;
+        oc.add(4);
+        const bool tir = fresnel(P.n_crystal, P.n_coupling, cosi, R);
+        bool refl = tir;
+        if (!tir) refl = randu<COUNT>(r, P, oc) < R;
+        if (refl) {
+            specular(p.ux, p.uy, p.uz, face);
+        } else {
+            const float ex = p.x - P.cx, ey = p.y - P.cy;
+            if (!(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2) {
+                if (randu<COUNT>(r, P, oc) <= P.pde) {
+                    int ii = 1 + (int)(p.x / P.pitch);
+                    int jj = 1 + (int)(p.y / P.pitch);
+                    ii = ii < P.nx ? ii : P.nx;
+                    jj = jj < P.ny ? jj : P.ny;
+                    const int l = (jj - 1) * P.nx + ii;
+                    lin = face == F_FRONT ? -l : l;
+                    oc.add(8);
+                    return ST_DETECTED;
+                }
+                return ST_ABS_SIPM;
+            } else if (randu<COUNT>(r, P, oc) <= P.sur_R) {
+                if (P.flags & K_SUR_SPEC) specular(p.ux, p.uy, p.uz, face);
+                else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+            } else {
+                return ST_ABS_SUR;
+            }
+        }
+    } else if (!(P.flags & K_AIR_GAP)) {
+        // front-painted wrap in optical contact (src/kernel.jl:252-262)
+        if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
+            if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
+            else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+        } else {
+            return ST_ABS_WALL;
+        }
+    } else {
+        // back-painted: crystal -> air, then the reflector (src/kernel.jl:263-293)
+        bool transmitted;
+        if (P.sigma_alpha > 0.0f) {
+            transmitted = unified_surface<COUNT>(r, P, oc, p.ux, p.uy, p.uz, face);
+        } else {
+            float R;
+            oc.add(4);
+            const bool tir = fresnel(P.n_crystal, 1.0f, cosi, R);
+            bool refl = tir;
+            if (!tir) refl = randu<COUNT>(r, P, oc) < R;
+            transmitted = !refl;
+            if (refl) specular(p.ux, p.uy, p.uz, face);
+        }
+        if (transmitted) {
+            if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
+                if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
+                else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+            } else {
+                return ST_ABS_WALL;
+            }
+        }
+    }
+    ++p.bounces;
+    return p.bounces >= P.maxb ? ST_CAP : result;
+}
+
+// Order-preserving u32 key of a float, so atomicMin on keys is min on floats.
+__device__ __forceinline__ uint32_t time_key(float t)
+{
+    const uint32_t b = __float_as_uint(t);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float key_time(uint32_t k)
+{
+    const uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    union { uint32_t u; float f; } v;
+    v.u = b;
+    return v.f;
+}
+constexpr uint32_t KEY_INF = 0xFF800000u;     // time_key(+Inf)
+
+}  // namespace crysp

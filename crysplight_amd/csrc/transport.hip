@@ -1,0 +1,925 @@
+// transport.hip -- gfx950 kernels of the optical-photon transport and the C-ABI
+// (include/crysplight.h) of libcrysplight_hip.so.
+//
+// Kernel design (one persistent kernel, three output modes):
+//  * Photons are numbered k = 0..n-1 inside a call.  A wave grabs CHUNK consecutive
+//    photon numbers at a time from one device cursor (one atomic per chunk) and hands
+//    them to its lanes as they free up: a lane whose photon ended takes a new photon
+//    at the top of the next step, so no lane idles behind the wave's longest photon.
+//  * Each loop iteration is ONE bounce step (crysp::step) for every live lane.
+//  * Records mode writes the reference's four per-photon records
+//    (src/kernel.jl:300-303).  Accumulate mode histograms detections in LDS per
+//    workgroup (counts + first-time keys) and flushes once with global atomics;
+//    counters are reduced per wave.  Event mode adds each detection to its event's
+//    Int32 map with a global atomic and first-hit time with atomicMin on an
+//    order-preserving key.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "crysp_device.h"
+#include "crysplight.h"
+
+using namespace crysp;
+
+namespace {
+
+enum Mode : int { M_RECORDS = 0, M_ACCUM = 1, M_EVENTS = 2 };
+
+struct KBuffers {
+    const float *px, *py, *pz, *pt;                 // records / accumulate inputs
+    const float4 *dep;                              // events: (x, y, z, t0) per deposit
+    const int64_t *dep_off;                         // events: ndep+1 photon offsets
+    const int32_t *dep_event;                       // events: event of each deposit
+    uint8_t *status; int16_t *idx; float *tdet; int32_t *bnc;   // records
+    uint32_t *counts; uint32_t *tkeys; unsigned long long *counters;  // accumulate
+    int32_t *maps; uint32_t *ev_tkeys;              // events
+    unsigned long long *cursor;                     // work-queue cursor (zeroed)
+    unsigned long long *opcounts;                   // 10 op counters (COUNT only)
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+// deposit holding photon g: largest d with dep_off[d] <= g
+__device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep, int64_t g)
+{
+    int64_t lo = 0, hi = ndep;                      // invariant off[lo] <= g < off[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int MODE, bool COUNT>
+__global__ void __launch_bounds__(1024)
+transport_kernel(const KParams P, const KBuffers B)
+{
+    extern __shared__ uint32_t lds[];               // accumulate: hist + time keys
+    const int nxy = P.nx * P.ny;
+    if (MODE == M_ACCUM) {
+        for (int w = threadIdx.x; w < P.hist_words + nxy; w += blockDim.x)
+            lds[w] = w < P.hist_words ? 0u : KEY_INF;
+        __syncthreads();
+    }
+
+    OpCount<COUNT> oc;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) oc.v[k] = 0u;
+
+    unsigned long long c_det = 0, c_front = 0, c_bulk = 0, c_wall = 0, c_sipm = 0, c_cap = 0,
+                       c_sur = 0, c_scat = 0, c_bsum = 0;
+
+    Photon p{};
+    Rng rng;
+    rng_init(rng, 0);
+    int64_t k = 0;                                  // photon number of this lane
+    int32_t ev = 0;                                 // its event (event mode)
+    int nscat = 0;
+    bool alive = false;
+
+    uint64_t pool_next = 0, pool_end = 0;           // wave-uniform work pool
+    bool exhausted = false;
+    const uint64_t N = (uint64_t)P.n_photons;
+
+    for (;;) {
+        // ---- refill lanes whose photon ended --------------------------------------
+        const uint64_t need = __ballot(!alive);
+        if (need != 0ull && !exhausted) {
+            const uint32_t nneed = (uint32_t)__popcll(need);
+            const uint32_t rank = lanes_below(need);
+            const uint64_t avail = pool_end - pool_next;
+            uint64_t base2 = 0, end2 = 0;
+            if ((uint64_t)nneed > avail) {
+                uint64_t b = 0;
+                if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id()))
+                    b = atomicAdd(B.cursor, (unsigned long long)P.chunk);
+                b = first_lane_u64(b);
+                base2 = b;
+                end2 = b + (uint64_t)P.chunk < N ? b + (uint64_t)P.chunk : N;
+                if (b >= N) { base2 = end2 = N; exhausted = true; }
+            }
+            bool take = false;
+            if (!alive) {
+                if ((uint64_t)rank < avail) { k = (int64_t)(pool_next + rank); take = true; }
+                else {
+                    const uint64_t r2 = base2 + ((uint64_t)rank - avail);
+                    if (r2 < end2) { k = (int64_t)r2; take = true; }
+                }
+            }
+            if ((uint64_t)nneed <= avail) pool_next += nneed;
+            else {
+                const uint64_t nx2 = base2 + ((uint64_t)nneed - avail);
+                pool_next = nx2 < end2 ? nx2 : end2;
+                pool_end = end2;
+            }
+            if (take) {
+                float x0, y0, z0, t0;
+                if (MODE == M_EVENTS) {
+                    const int64_t d = find_deposit(B.dep_off, P.ndep, k);
+                    const float4 q = B.dep[d];
+                    x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
+                    ev = B.dep_event[d];
+                } else {
+                    const int64_t j = P.npos == 1 ? 0 : k % (int64_t)P.npos;
+                    x0 = B.px[j]; y0 = B.py[j]; z0 = B.pz[j];
+                    t0 = B.pt ? B.pt[j] : 0.0f;
+                }
+                rng_init(rng, P.pid0 + (uint64_t)k + 1ull);
+                p.x = x0; p.y = y0; p.z = z0; p.bounces = 0;
+                emit<COUNT>(rng, P, oc, t0, p.ux, p.uy, p.uz, p.t);
+                nscat = 0;
+                alive = true;
+                oc.add(0);
+            }
+        }
+        if (__ballot(alive) == 0ull) break;         // queue drained, all lanes idle
+
+        // ---- one bounce step ------------------------------------------------------
+        if (alive) {
+            int lin = 0;
+            const int st = step<COUNT>(p, rng, P, oc, lin, nscat);
+            if (st != ST_FLYING) {
+                alive = false;
+                if (MODE == M_RECORDS) {
+                    if (B.status) B.status[k] = (uint8_t)st;
+                    if (B.idx) B.idx[k] = (int16_t)(st == ST_DETECTED ? lin : 0);
+                    if (B.tdet) B.tdet[k] = p.t;
+                    if (B.bnc) B.bnc[k] = p.bounces;
+                } else if (MODE == M_ACCUM) {
+                    c_scat += (unsigned)nscat;
+                    if (st == ST_DETECTED) {
+                        int l = lin;
+                        if (l < 0) { ++c_front; l = -l; }
+                        const int cell = l - 1;     // (ii-1) + nx*(jj-1)
+                        const float q = p.t / P.bin_ns;
+                        int bin = (q >= (float)P.nbins) ? P.nbins : (int)q;  // 0-based
+                        bin = bin < 0 ? 0 : bin;
+                        atomicAdd(&lds[cell + nxy * bin], 1u);
+                        atomicMin(&lds[P.hist_words + cell], time_key(p.t));
+                        ++c_det;
+                        c_bsum += (unsigned)p.bounces;
+                    } else if (st == ST_ABS_BULK) ++c_bulk;
+                    else if (st == ST_ABS_WALL) ++c_wall;
+                    else if (st == ST_ABS_SIPM) ++c_sipm;
+                    else if (st == ST_ABS_SUR) ++c_sur;
+                    else ++c_cap;
+                } else {
+                    if (st == ST_DETECTED) {
+                        const int l = lin < 0 ? -lin : lin;
+                        const int64_t cell = (int64_t)(l - 1) + (int64_t)nxy * ev;
+                        atomicAdd(&B.maps[cell], 1);
+                        atomicMin(&B.ev_tkeys[cell], time_key(p.t));
+                    }
+                }
+            }
+        }
+    }
+
+    if (COUNT) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+            const unsigned long long s = wave_sum((unsigned long long)oc.v[q]);
+            if (__lane_id() == 0 && s) atomicAdd(&B.opcounts[q], s);
+        }
+    }
+    if (MODE == M_ACCUM) {
+        unsigned long long vals[9] = { c_det, c_front, c_bulk, c_wall, c_sipm, c_cap, c_sur,
+                                       c_scat, c_bsum };
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const unsigned long long s = wave_sum(vals[q]);
+            if (__lane_id() == 0 && s) atomicAdd(&B.counters[q], s);
+        }
+        __syncthreads();
+        for (int w = threadIdx.x; w < P.hist_words; w += blockDim.x)
+            if (lds[w]) atomicAdd(&B.counts[w], lds[w]);
+        for (int w = threadIdx.x; w < nxy; w += blockDim.x)
+            if (lds[P.hist_words + w] != KEY_INF) atomicMin(&B.tkeys[w], lds[P.hist_words + w]);
+    }
+}
+
+// -------------------------------------------------------------- small kernels ----
+__global__ void fill_u32(uint32_t *p, uint32_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void keys_to_times(const uint32_t *keys, float *t, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = key_time(keys[i]);
+}
+
+// per event: npe = sum of its map (src/events.jl:86), tmin from keys
+__global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *tmin,
+                              int32_t *npe, int64_t nev, int nxy)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nev;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t s = 0;
+        for (int c = 0; c < nxy; ++c) {
+            s += maps[e * nxy + c];
+            tmin[e * nxy + c] = key_time(keys[e * nxy + c]);
+        }
+        npe[e] = s;
+    }
+}
+
+__global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids, int n_draws,
+                                   uint32_t *out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n_pids) return;
+    KParams P{};
+    P.key0 = (uint32_t)seed; P.key1 = (uint32_t)(seed >> 32);
+    Rng r;
+    rng_init(r, pid0 + (uint64_t)i + 1ull);
+    OpCount<false> oc;
+    for (int d = 0; d < n_draws; ++d) out[i * n_draws + d] = rng_next<false>(r, P, oc);
+}
+
+__global__ void debug_math_kernel(int which, const float *x, int64_t n, float *y)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = x[i];
+        float s, c;
+        if (which == 0) y[i] = f_log(v);
+        else if (which == 3) y[i] = sqrtf(v);
+        else if (which == 4) y[i] = f_cos(v);
+        else { f_sincos(v, s, c); y[i] = which == 1 ? s : c; }
+    }
+}
+
+}  // namespace
+
+// =============================================================== host side ====
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(CRYSP_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct crysp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    int max_threads_per_cu = 2048;
+    size_t lds_per_block = 65536;
+    int wg = 256;
+    int ppt = 4;
+    int chunk = 256;
+    bool timing = false;
+    double kernel_ms = 0.0;
+    int64_t n_launches = 0;
+    bool count_ops = false;
+    unsigned long long ops_host[10] = {};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf bufs[24];
+    // captured event-mode graph
+    hipGraphExec_t graph = nullptr;
+    std::vector<unsigned char> graph_key;
+};
+
+enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
+             B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
+             B_NPE, B_OPS, B_MISC, B_MISC2, B_NBUF };
+
+static int ensure(crysp_ctx *c, int id, size_t bytes, void **out)
+{
+    DevBuf &b = c->bufs[id];
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr; b.cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&b.p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(CRYSP_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        b.cap = want;
+    }
+    *out = b.p;
+    return CRYSP_OK;
+}
+
+// true when p is device memory usable in place by kernels on the context's device
+static bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// input: device pointer in place, or a staged copy in buffer `id`
+static int stage_in(crysp_ctx *c, int id, const void *src, size_t bytes, const void **dev)
+{
+    if (!src) { *dev = nullptr; return CRYSP_OK; }
+    if (is_device_ptr(src)) { *dev = src; return CRYSP_OK; }
+    void *d;
+    int rc = ensure(c, id, bytes, &d);
+    if (rc) return rc;
+    if (bytes) HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *dev = d;
+    return CRYSP_OK;
+}
+
+// output: device pointer in place, or buffer `id` copied back by stage_out
+static int out_buf(crysp_ctx *c, int id, void *dst, size_t bytes, void **dev)
+{
+    if (!dst) { *dev = nullptr; return CRYSP_OK; }
+    if (is_device_ptr(dst)) { *dev = dst; return CRYSP_OK; }
+    return ensure(c, id, bytes, dev);
+}
+
+static int stage_out(crysp_ctx *c, void *dst, const void *dev, size_t bytes)
+{
+    if (!dst || dst == dev || !bytes) return CRYSP_OK;
+    HIPCHK(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    return CRYSP_OK;
+}
+
+static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_params *op,
+                       const crysp_readout *ro, const crysp_emission *em, uint64_t seed,
+                       uint64_t pid0, int32_t maxb)
+{
+    if (!box || !op || !ro) return fail(CRYSP_EINVAL, "null parameter struct");
+    if (ro->grid.nx < 1 || ro->grid.ny < 1)
+        return fail(CRYSP_EINVAL, "SiPM grid must have nx, ny >= 1");
+    if ((int64_t)ro->grid.nx * ro->grid.ny > 32767)
+        return fail(CRYSP_ERANGE, "nx*ny > 32767 overflows the Int16 SiPM index");
+    if (!(ro->grid.pitch_mm > 0.0f)) return fail(CRYSP_EINVAL, "pitch_mm must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(box->L[a] > 0.0f) || !std::isfinite(box->L[a]))
+            return fail(CRYSP_EINVAL, "box sides must be finite and > 0");
+    if (maxb < 1) return fail(CRYSP_EINVAL, "max_bounces must be >= 1");
+    std::memset(&P, 0, sizeof P);
+    P.L0 = box->L[0]; P.L1 = box->L[1]; P.L2 = box->L[2];
+    P.n_crystal = op->n_crystal; P.n_coupling = op->n_coupling;
+    P.abs_len = op->abs_len_mm; P.ray_len = op->rayleigh_mm;
+    P.wrap_R = op->wrap_R; P.sigma_alpha = op->sigma_alpha; P.pde = op->pde;
+    P.inv_v = op->n_crystal / C_MM_NS;
+    P.cx = ro->cx; P.cy = ro->cy; P.r2 = ro->r2; P.sur_R = ro->sur_R;
+    P.pitch = ro->grid.pitch_mm; P.nx = ro->grid.nx; P.ny = ro->grid.ny;
+    P.tau_d = em ? em->tau_decay : 0.0f;
+    P.tau_r = em ? em->tau_rise : 0.0f;
+    P.maxb = maxb;
+    uint32_t f = 0;
+    if (std::isfinite(op->abs_len_mm)) f |= K_ABS;
+    if (std::isfinite(op->rayleigh_mm)) f |= K_RAY;
+    if (op->wrap_specular) f |= K_WRAP_SPEC;
+    if (op->air_gap) f |= K_AIR_GAP;
+    if (ro->disc) f |= K_DISC;
+    if (ro->sur_specular) f |= K_SUR_SPEC;
+    if (ro->two_sided) f |= K_TWO_SIDED;
+    P.flags = f;
+    P.key0 = (uint32_t)seed; P.key1 = (uint32_t)(seed >> 32);
+    P.pid0 = pid0;
+    return CRYSP_OK;
+}
+
+static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
+{
+    // persistent grid: ceil(n / (wg*ppt)) workgroups, at most what stays resident
+    int64_t want = (n + (int64_t)wg * ppt - 1) / ((int64_t)wg * ppt);
+    int per_cu = std::max(1, c->max_threads_per_cu / wg);
+    if (lds_bytes) per_cu = std::max(1, std::min<int>(per_cu, (int)(160 * 1024 / lds_bytes)));
+    int64_t cap = (int64_t)c->n_cu * per_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
+}
+
+template <int MODE>
+static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_bytes)
+{
+    const int wg = c->wg;
+    const int grid = grid_for(c, P.n_photons, wg, c->ppt, lds_bytes);
+    P.chunk = c->chunk;
+    void *cur;
+    int rc = ensure(c, B_CURSOR, sizeof(unsigned long long), &cur);
+    if (rc) return rc;
+    B.cursor = (unsigned long long *)cur;
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
+    if (c->count_ops) {
+        void *ops;
+        rc = ensure(c, B_OPS, 10 * sizeof(unsigned long long), &ops);
+        if (rc) return rc;
+        B.opcounts = (unsigned long long *)ops;
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)ops, 0u, 20);
+    }
+    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (c->count_ops)
+        hipLaunchKernelGGL((transport_kernel<MODE, true>), dim3(grid), dim3(wg), lds_bytes,
+                           c->stream, P, B);
+    else
+        hipLaunchKernelGGL((transport_kernel<MODE, false>), dim3(grid), dim3(wg), lds_bytes,
+                           c->stream, P, B);
+    HIPCHK(hipGetLastError());
+    if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return CRYSP_OK;
+}
+
+static int finish_timing(crysp_ctx *c)
+{
+    if (c->count_ops) {
+        unsigned long long h[10];
+        HIPCHK(hipMemcpyAsync(h, c->bufs[B_OPS].p, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < 10; ++q) c->ops_host[q] += h[q];
+    }
+    if (!c->timing) return CRYSP_OK;
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->kernel_ms += ms;
+    c->n_launches += 1;
+    return CRYSP_OK;
+}
+
+extern "C" {
+
+const char *crysp_last_error(void) { return g_err.c_str(); }
+int crysp_abi_version(void) { return CRYSPLIGHT_ABI_VERSION; }
+
+crysp_ctx *crysp_create(int device)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        fail(CRYSP_EHIP, "no HIP device available");
+        return nullptr;
+    }
+    if (device < 0 || device >= ndev) {
+        fail(CRYSP_EINVAL, "device index out of range");
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        fail(CRYSP_EHIP, "hipSetDevice failed");
+        return nullptr;
+    }
+    crysp_ctx *c = new crysp_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        fail(CRYSP_EHIP, "hipGetDeviceProperties failed");
+        return nullptr;
+    }
+    c->n_cu = prop.multiProcessorCount;
+    c->max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        fail(CRYSP_EHIP, "stream/event creation failed");
+        return nullptr;
+    }
+    return c;
+}
+
+void crysp_destroy(crysp_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (auto &b : c->bufs)
+        if (b.p) (void)hipFree(b.p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int crysp_set_schedule(crysp_ctx *c, int32_t wg, int32_t ppt)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (wg < 64 || wg > 1024 || wg % 64) return fail(CRYSP_EINVAL, "wg_size must be 64..1024, multiple of 64");
+    if (ppt < 1 || ppt > 64) return fail(CRYSP_EINVAL, "photons_per_thread must be 1..64");
+    c->wg = wg;
+    c->ppt = ppt;
+    return CRYSP_OK;
+}
+
+int crysp_get_schedule(crysp_ctx *c, int32_t *wg, int32_t *ppt)
+{
+    if (!c || !wg || !ppt) return fail(CRYSP_EINVAL, "null argument");
+    *wg = c->wg;
+    *ppt = c->ppt;
+    return CRYSP_OK;
+}
+
+int crysp_set_timing(crysp_ctx *c, int32_t enable)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    c->timing = enable != 0;
+    return CRYSP_OK;
+}
+
+int crysp_kernel_time(crysp_ctx *c, double *total_ms, int64_t *n, int32_t reset)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (total_ms) *total_ms = c->kernel_ms;
+    if (n) *n = c->n_launches;
+    if (reset) { c->kernel_ms = 0.0; c->n_launches = 0; }
+    return CRYSP_OK;
+}
+
+int crysp_set_op_counting(crysp_ctx *c, int32_t enable)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    c->count_ops = enable != 0;
+    return CRYSP_OK;
+}
+
+int crysp_get_op_counts(crysp_ctx *c, int64_t *out, int32_t reset)
+{
+    if (!c || !out) return fail(CRYSP_EINVAL, "null argument");
+    for (int q = 0; q < 10; ++q) out[q] = (int64_t)c->ops_host[q];
+    if (reset) std::memset(c->ops_host, 0, sizeof c->ops_host);
+    return CRYSP_OK;
+}
+
+int crysp_transport_records(crysp_ctx *c, const crysp_box *box,
+                            const crysp_optical_params *op, const crysp_readout *ro,
+                            const float *px, const float *py, const float *pz,
+                            const float *pt, int32_t npos, int64_t n, uint64_t seed,
+                            uint64_t pid0, int32_t maxb, const crysp_emission *em,
+                            uint8_t *status, int16_t *idx, float *tdet, int32_t *bnc)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (n < 0) return fail(CRYSP_EINVAL, "n_photons < 0");
+    if (npos < 1 || !px || !py || !pz) return fail(CRYSP_EINVAL, "need npos >= 1 and px, py, pz");
+    KParams P;
+    int rc = fill_params(P, box, op, ro, em, seed, pid0, maxb);
+    if (rc) return rc;
+    if (n == 0) return CRYSP_OK;
+    HIPCHK(hipSetDevice(c->device));
+    P.n_photons = n;
+    P.npos = npos;
+    KBuffers B{};
+    const size_t np = (size_t)npos * sizeof(float);
+    if ((rc = stage_in(c, B_PX, px, np, (const void **)&B.px)) ||
+        (rc = stage_in(c, B_PY, py, np, (const void **)&B.py)) ||
+        (rc = stage_in(c, B_PZ, pz, np, (const void **)&B.pz)) ||
+        (rc = stage_in(c, B_PT, pt, np, (const void **)&B.pt)))
+        return rc;
+    if ((rc = out_buf(c, B_STATUS, status, (size_t)n, (void **)&B.status)) ||
+        (rc = out_buf(c, B_IDX, idx, (size_t)n * 2, (void **)&B.idx)) ||
+        (rc = out_buf(c, B_TDET, tdet, (size_t)n * 4, (void **)&B.tdet)) ||
+        (rc = out_buf(c, B_BNC, bnc, (size_t)n * 4, (void **)&B.bnc)))
+        return rc;
+    if ((rc = launch_transport<M_RECORDS>(c, P, B, 0))) return rc;
+    if ((rc = stage_out(c, status, B.status, (size_t)n)) ||
+        (rc = stage_out(c, idx, B.idx, (size_t)n * 2)) ||
+        (rc = stage_out(c, tdet, B.tdet, (size_t)n * 4)) ||
+        (rc = stage_out(c, bnc, B.bnc, (size_t)n * 4)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return finish_timing(c);
+}
+
+int crysp_transport_accumulate(crysp_ctx *c, const crysp_box *box,
+                               const crysp_optical_params *op, const crysp_readout *ro,
+                               const crysp_time_binning *tb, const float *px, const float *py,
+                               const float *pz, const float *pt, int32_t npos, int64_t n,
+                               uint64_t seed, uint64_t pid0, int32_t maxb,
+                               const crysp_emission *em, uint32_t *counts, float *first_ns,
+                               int64_t *counters)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (n < 0) return fail(CRYSP_EINVAL, "n_photons < 0");
+    if (!tb || tb->nbins < 0 || !(tb->bin_ns > 0.0f)) return fail(CRYSP_EINVAL, "bad time binning");
+    if (npos < 1 || !px || !py || !pz) return fail(CRYSP_EINVAL, "need npos >= 1 and px, py, pz");
+    if (!counts || !first_ns || !counters) return fail(CRYSP_EINVAL, "null output");
+    KParams P;
+    int rc = fill_params(P, box, op, ro, em, seed, pid0, maxb);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t nxy = (int64_t)ro->grid.nx * ro->grid.ny;
+    const int64_t hist = nxy * ((int64_t)tb->nbins + 1);
+    const size_t lds_bytes = (size_t)(hist + nxy) * 4;
+    if (lds_bytes > 64 * 1024)
+        return fail(CRYSP_ERANGE, "nx*ny*(nbins+2) words exceed the 64 KiB workgroup LDS histogram");
+    P.n_photons = n;
+    P.npos = npos;
+    P.nbins = tb->nbins;
+    P.bin_ns = tb->bin_ns;
+    P.hist_words = (int32_t)hist;
+    KBuffers B{};
+    const size_t np = (size_t)npos * sizeof(float);
+    if ((rc = stage_in(c, B_PX, px, np, (const void **)&B.px)) ||
+        (rc = stage_in(c, B_PY, py, np, (const void **)&B.py)) ||
+        (rc = stage_in(c, B_PZ, pz, np, (const void **)&B.pz)) ||
+        (rc = stage_in(c, B_PT, pt, np, (const void **)&B.pt)))
+        return rc;
+    void *dcounts, *dkeys, *dctr, *dfirst;
+    if ((rc = out_buf(c, B_COUNTS, counts, (size_t)hist * 4, &dcounts)) ||
+        (rc = ensure(c, B_TKEYS, (size_t)nxy * 4, &dkeys)) ||
+        (rc = ensure(c, B_COUNTERS, 16 * 8, &dctr)) ||
+        (rc = out_buf(c, B_TMIN, first_ns, (size_t)nxy * 4, &dfirst)))
+        return rc;
+    B.counts = (uint32_t *)dcounts;
+    B.tkeys = (uint32_t *)dkeys;
+    B.counters = (unsigned long long *)dctr;
+    hipLaunchKernelGGL(fill_u32, dim3(64), dim3(256), 0, c->stream, B.counts, 0u, hist);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, c->stream, B.tkeys, KEY_INF, nxy);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)dctr, 0u, 32);
+    if (n > 0 && (rc = launch_transport<M_ACCUM>(c, P, B, lds_bytes))) return rc;
+    hipLaunchKernelGGL(keys_to_times, dim3(1), dim3(256), 0, c->stream, B.tkeys,
+                       (float *)dfirst, nxy);
+    unsigned long long hc[16];
+    HIPCHK(hipMemcpyAsync(hc, dctr, sizeof hc, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = stage_out(c, counts, dcounts, (size_t)hist * 4)) ||
+        (rc = stage_out(c, first_ns, dfirst, (size_t)nxy * 4)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < CRYSP_NCOUNTERS; ++q) counters[q] = (int64_t)hc[q];
+    if (n > 0) return finish_timing(c);
+    return CRYSP_OK;
+}
+
+// Event mode with the launch sequence captured once in a hipGraph and replayed while
+// the arguments (pointers, sizes, parameters) stay the same.
+static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                       const crysp_readout *ro, const float *dep_xyzt, const int32_t *dep_nph,
+                       const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
+                       uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
+                       float *tmin, int32_t *npe, int64_t *n_out, bool use_graph)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (ndep < 0 || nev < 0) return fail(CRYSP_EINVAL, "negative sizes");
+    if (ndep > 0 && (!dep_xyzt || !dep_nph || !dep_event))
+        return fail(CRYSP_EINVAL, "null deposit arrays");
+    if (nev > 0 && (!maps || !tmin || !npe)) return fail(CRYSP_EINVAL, "null outputs");
+    KParams P;
+    int rc = fill_params(P, box, op, ro, em, seed, pid0, maxb);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const int nxy = ro->grid.nx * ro->grid.ny;
+    // deposit table: photon offsets are computed on the host when the counts live there
+    int64_t ntot = 0;
+    const bool nph_dev = is_device_ptr(dep_nph);
+    const bool ev_dev = is_device_ptr(dep_event);
+    std::vector<int32_t> hnph, hev;
+    const int32_t *nph_h = dep_nph, *ev_h = dep_event;
+    if (nph_dev && ndep) {
+        hnph.resize(ndep);
+        HIPCHK(hipMemcpy(hnph.data(), dep_nph, ndep * 4, hipMemcpyDeviceToHost));
+        nph_h = hnph.data();
+    }
+    if (ev_dev && ndep) {
+        hev.resize(ndep);
+        HIPCHK(hipMemcpy(hev.data(), dep_event, ndep * 4, hipMemcpyDeviceToHost));
+        ev_h = hev.data();
+    }
+    std::vector<int64_t> off(ndep + 1, 0);
+    for (int64_t d = 0; d < ndep; ++d) {
+        if (nph_h[d] < 0) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
+        if (ev_h[d] < 0 || ev_h[d] >= nev) return fail(CRYSP_EINVAL, "dep_event out of range");
+        off[d + 1] = off[d] + nph_h[d];
+    }
+    ntot = off[ndep];
+    if (n_out) *n_out = ntot;
+    P.n_photons = ntot;
+    P.ndep = ndep;
+    KBuffers B{};
+    void *doff;
+    if ((rc = ensure(c, B_DEPOFF, (size_t)(ndep + 1) * 8, &doff))) return rc;
+    HIPCHK(hipMemcpyAsync(doff, off.data(), (size_t)(ndep + 1) * 8, hipMemcpyHostToDevice,
+                          c->stream));
+    B.dep_off = (const int64_t *)doff;
+    if ((rc = stage_in(c, B_DEP, dep_xyzt, (size_t)ndep * 16, (const void **)&B.dep)) ||
+        (rc = stage_in(c, B_DEPEV, dep_event, (size_t)ndep * 4, (const void **)&B.dep_event)))
+        return rc;
+    void *dmaps, *dtmin, *dnpe, *dkeys;
+    const size_t cells = (size_t)nev * nxy;
+    if ((rc = out_buf(c, B_MAPS, maps, cells * 4, &dmaps)) ||
+        (rc = out_buf(c, B_TMIN, tmin, cells * 4, &dtmin)) ||
+        (rc = out_buf(c, B_NPE, npe, (size_t)nev * 4, &dnpe)) ||
+        (rc = ensure(c, B_EVKEYS, cells * 4, &dkeys)))
+        return rc;
+    B.maps = (int32_t *)dmaps;
+    B.ev_tkeys = (uint32_t *)dkeys;
+    void *cur;
+    if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
+    B.cursor = (unsigned long long *)cur;
+    P.chunk = c->chunk;
+    const int grid = grid_for(c, ntot, c->wg, c->ppt, 0);
+
+    // graph key: everything the captured launches depend on
+    std::vector<unsigned char> key(sizeof P + sizeof B + 4 * sizeof(int64_t));
+    {
+        int64_t extra[4] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops };
+        unsigned char *q = key.data();
+        std::memcpy(q, &P, sizeof P); q += sizeof P;
+        std::memcpy(q, &B, sizeof B); q += sizeof B;
+        std::memcpy(q, extra, sizeof extra);
+    }
+    const bool graphable = use_graph && !c->count_ops && !c->timing;
+    if (graphable && c->graph && key == c->graph_key) {
+        HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    } else {
+        auto enqueue = [&]() -> int {
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
+            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
+                               (int64_t)cells);
+            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
+                               KEY_INF, (int64_t)cells);
+            if (ntot > 0) {
+                if (c->count_ops) {
+                    void *ops;
+                    int r2 = ensure(c, B_OPS, 80, &ops);
+                    if (r2) return r2;
+                    B.opcounts = (unsigned long long *)ops;
+                    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)ops, 0u, 20);
+                }
+                if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
+                if (c->count_ops)
+                    hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg),
+                                       0, c->stream, P, B);
+                else
+                    hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg),
+                                       0, c->stream, P, B);
+                if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
+            }
+            if (nev > 0)
+                hipLaunchKernelGGL(finish_events, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
+                                   dim3(256), 0, c->stream, (const int32_t *)dmaps,
+                                   (const uint32_t *)dkeys, (float *)dtmin, (int32_t *)dnpe, nev, nxy);
+            return CRYSP_OK;
+        };
+        if (graphable) {
+            if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue();
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(CRYSP_EHIP, "hipStreamEndCapture failed");
+            HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            c->graph_key = key;
+            HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        } else {
+            if ((rc = enqueue())) return rc;
+            HIPCHK(hipGetLastError());
+        }
+    }
+    if ((rc = stage_out(c, maps, dmaps, cells * 4)) ||
+        (rc = stage_out(c, tmin, dtmin, cells * 4)) ||
+        (rc = stage_out(c, npe, dnpe, (size_t)nev * 4)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ntot > 0 && !graphable) return finish_timing(c);
+    return CRYSP_OK;
+}
+
+int crysp_run_events(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                     const crysp_readout *ro, const float *dep_xyzt, const int32_t *dep_nph,
+                     const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
+                     uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
+                     float *tmin, int32_t *npe, int64_t *n_out)
+{
+    return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
+                       maxb, em, maps, tmin, npe, n_out, true);
+}
+
+int crysp_autotune_events(crysp_ctx *c, const crysp_box *box,
+                          const crysp_optical_params *op, const crysp_readout *ro,
+                          const float *dep_xyzt, const int32_t *dep_nph,
+                          const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
+                          int32_t maxb, const crysp_emission *em, const int32_t *wgs,
+                          int32_t n_wg, const int32_t *ppts, int32_t n_ppt, int32_t reps,
+                          float *times_ms, int32_t *best_wg, int32_t *best_ppt)
+{
+    if (!c || !wgs || !ppts || n_wg < 1 || n_ppt < 1 || reps < 1)
+        return fail(CRYSP_EINVAL, "bad autotune arguments");
+    const int old_wg = c->wg, old_ppt = c->ppt;
+    const bool old_timing = c->timing;
+    const double old_ms = c->kernel_ms;
+    const int64_t old_n = c->n_launches;
+    const int nxy = ro ? ro->grid.nx * ro->grid.ny : 0;
+    std::vector<int32_t> hm((size_t)std::max<int64_t>(1, nev) * std::max(1, nxy)),
+        hn((size_t)std::max<int64_t>(1, nev));
+    std::vector<float> ht(hm.size());
+    // outputs go to device scratch so the timing is the kernel's own
+    void *dm, *dt;
+    int rc;
+    if ((rc = ensure(c, B_MISC, hm.size() * 4, &dm)) || (rc = ensure(c, B_MISC2, ht.size() * 4, &dt)))
+        return rc;
+    void *dnpe;
+    if ((rc = ensure(c, B_NPE, hn.size() * 4, &dnpe))) return rc;
+    float best = 1e30f;
+    int bw = old_wg, bp = old_ppt;
+    c->timing = true;
+    for (int a = 0; a < n_wg; ++a) {
+        for (int b = 0; b < n_ppt; ++b) {
+            if ((rc = crysp_set_schedule(c, wgs[a], ppts[b]))) goto done;
+            float tbest = 1e30f;
+            for (int r = 0; r < reps; ++r) {
+                c->kernel_ms = 0.0;
+                c->n_launches = 0;
+                rc = events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed,
+                                 0, maxb, em, (int32_t *)dm, (float *)dt, (int32_t *)dnpe,
+                                 nullptr, false);
+                if (rc) goto done;
+                tbest = std::min(tbest, (float)c->kernel_ms);
+            }
+            if (times_ms) times_ms[a * n_ppt + b] = tbest;
+            if (tbest < best) { best = tbest; bw = wgs[a]; bp = ppts[b]; }
+        }
+    }
+done:
+    c->timing = old_timing;
+    c->kernel_ms = old_ms;
+    c->n_launches = old_n;
+    if (rc) { c->wg = old_wg; c->ppt = old_ppt; return rc; }
+    c->wg = bw;
+    c->ppt = bp;
+    if (best_wg) *best_wg = bw;
+    if (best_ppt) *best_ppt = bp;
+    return CRYSP_OK;
+}
+
+int crysp_debug_draws(crysp_ctx *c, uint64_t seed, uint64_t pid0, int64_t n_pids,
+                      int32_t n_draws, uint32_t *out)
+{
+    if (!c || !out || n_pids < 0 || n_draws < 0) return fail(CRYSP_EINVAL, "bad arguments");
+    if (n_pids == 0 || n_draws == 0) return CRYSP_OK;
+    HIPCHK(hipSetDevice(c->device));
+    void *d;
+    int rc = out_buf(c, B_MISC, out, (size_t)n_pids * n_draws * 4, &d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(debug_draws_kernel, dim3((unsigned)((n_pids + 255) / 256)), dim3(256), 0,
+                       c->stream, seed, pid0, n_pids, n_draws, (uint32_t *)d);
+    HIPCHK(hipGetLastError());
+    if ((rc = stage_out(c, out, d, (size_t)n_pids * n_draws * 4))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CRYSP_OK;
+}
+
+int crysp_debug_math(crysp_ctx *c, int32_t which, const float *x, int64_t n, float *y)
+{
+    if (!c || !x || !y || n < 0 || which < 0 || which > 4) return fail(CRYSP_EINVAL, "bad arguments");
+    if (n == 0) return CRYSP_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const float *dx;
+    void *dy;
+    int rc;
+    if ((rc = stage_in(c, B_MISC, x, (size_t)n * 4, (const void **)&dx)) ||
+        (rc = out_buf(c, B_MISC2, y, (size_t)n * 4, &dy)))
+        return rc;
+    hipLaunchKernelGGL(debug_math_kernel, dim3(2048), dim3(256), 0, c->stream, which, dx, n,
+                       (float *)dy);
+    HIPCHK(hipGetLastError());
+    if ((rc = stage_out(c, y, dy, (size_t)n * 4))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CRYSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+/*
+ * crysplight.h -- C-ABI of libcrysplight_hip.so, the MI355X (gfx950) optical-photon
+ * transport of CryspLight.
+ *
+ * Conventions
+ *  - Every entry point returns 0 (CRYSP_OK) or a negative crysp_error; nothing throws
+ *    across the ABI.  crysp_last_error() gives the message of the calling thread's
+ *    last failure.
+ *  - Array arguments are plain pointers.  Each one may be host memory (pageable or
+ *    pinned; the library stages it through its own device buffers) or device memory
+ *    from hipMalloc / a torch tensor on the context's device (used in place).  All
+ *    calls are synchronous, as the reference's KA.synchronize (src/kernel.jl:335).
+ *  - Photon ids follow the reference: photon k (1-based within a call) draws from the
+ *    Philox stream (seed, pid = k + pid0) (src/kernel.jl:162-163).
+ *  - Multi-dimensional outputs are column-major like the Julia arrays they replace:
+ *    counts (nx, ny, nbins+1): element (i, j, b) (1-based) at (i-1) + nx*((j-1) + ny*(b-1));
+ *    per-event maps (nx, ny, nev): (i-1) + nx*(j-1) + nx*ny*(ev-1).
+ *  - One host thread per context at a time (the reference launches from one task).
+ */
+#ifndef CRYSPLIGHT_H
+#define CRYSPLIGHT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRYSPLIGHT_ABI_VERSION 1
+
+/* ---- parameter structs: byte-identical to the reference's isbits structs ---- */
+
+/* Box (src/geometry.jl:11-13): crystal [0,Lx]x[0,Ly]x[0,Lz] in mm, SiPMs at z = Lz. */
+typedef struct { float L[3]; } crysp_box;
+
+/* OpticalParams (src/transport.jl:14-24), 32 bytes: bools at offsets 20, 21. */
+typedef struct {
+    float   n_crystal;
+    float   n_coupling;      /* grease (or gap) index at the SiPM face               */
+    float   abs_len_mm;      /* bulk absorption length; +Inf disables (no draw)      */
+    float   rayleigh_mm;     /* Rayleigh length; +Inf disables (no draw)             */
+    float   wrap_R;          /* wrap reflectivity                                    */
+    uint8_t wrap_specular;   /* 1 = ESR (specular), 0 = PTFE (Lambertian)            */
+    uint8_t air_gap;         /* 1 = back-painted (air gap), 0 = front-painted        */
+    uint8_t pad_[2];
+    float   sigma_alpha;     /* UNIFIED roughness [rad] of the crystal-air interface */
+    float   pde;             /* photon detection efficiency at the readout face      */
+} crysp_optical_params;
+
+/* SipmGrid (src/transport.jl:26-30). */
+typedef struct { float pitch_mm; int32_t nx, ny; } crysp_sipm_grid;
+
+/* Readout (src/transport.jl:41-50), 36 bytes. */
+typedef struct {
+    crysp_sipm_grid grid;
+    uint8_t disc;            /* 1 = only a disc of radius^2 r2 about (cx, cy) is active */
+    uint8_t pad0_[3];
+    float   cx, cy, r2;
+    float   sur_R;           /* surround reflectivity outside the disc               */
+    uint8_t sur_specular;
+    uint8_t two_sided;       /* readout on both z faces                              */
+    uint8_t pad1_[2];
+} crysp_readout;
+
+/* TimeBinning (src/transport.jl:59-62): nbins + 1 bins, the last is overflow. */
+typedef struct { float bin_ns; int32_t nbins; } crysp_time_binning;
+
+/* Emission model.  tau_decay > 0: t += Exp(tau_decay) (one draw, src/kernel.jl:177-180).
+ * tau_rise > 0 (no reference counterpart): a second draw adds Exp(tau_rise), so the
+ * emission pdf is (e^{-t/td} - e^{-t/tr}) / (td - tr); tau_rise = 0 draws as the reference. */
+typedef struct { float tau_decay; float tau_rise; } crysp_emission;
+
+/* Photon fates (src/transport.jl:6-12). */
+enum crysp_status {
+    CRYSP_DETECTED = 1, CRYSP_ABS_BULK = 2, CRYSP_ABS_WALL = 3,
+    CRYSP_ABS_SIPM = 4, CRYSP_CAP = 5, CRYSP_ABS_SUR = 6
+};
+
+enum crysp_error {
+    CRYSP_OK = 0,
+    CRYSP_EINVAL = -1,   /* bad argument (null pointer, npos < 1, ...)               */
+    CRYSP_EHIP = -2,     /* a HIP runtime call failed                                */
+    CRYSP_ENOMEM = -3,   /* device allocation failed                                 */
+    CRYSP_ERANGE = -4    /* size outside what the ABI supports (nx*ny > 32767, ...)  */
+};
+
+/* Indices of the int64 counters[] array of crysp_transport_accumulate, the fields of
+ * the reference Accumulator (src/transport.jl:64-76). */
+enum crysp_counter {
+    CRYSP_C_NDET = 0, CRYSP_C_NDET_FRONT, CRYSP_C_NABS_BULK, CRYSP_C_NABS_WALL,
+    CRYSP_C_NABS_SIPM, CRYSP_C_NCAP, CRYSP_C_NABS_SUR, CRYSP_C_NSCAT,
+    CRYSP_C_SUM_BOUNCES_DET, CRYSP_NCOUNTERS
+};
+
+typedef struct crysp_ctx crysp_ctx;
+
+/* Context: owns a HIP stream on `device`, cached device buffers, captured hipGraphs
+ * and the autotuned launch schedule.  Replaces the ArrayT backend selector
+ * (src/kernel.jl:319,330; scripts/shoot_gammas.jl:47). NULL on failure. */
+crysp_ctx *crysp_create(int device);
+void crysp_destroy(crysp_ctx *ctx);
+const char *crysp_last_error(void);
+int crysp_abi_version(void);
+
+/* Records mode: bit-for-bit replacement of transport_kernel!
+ * (src/kernel.jl:159-305) over ndrange = n_photons.  Photon k (1-based) starts at
+ * position j = 1 + (k-1) % npos with t0 = pt[j] (pt may be NULL: zeros), draws an
+ * isotropic direction and the emission delay, and is transported until it ends.
+ * Outputs per photon: status (crysp_status), idx (SiPM linear index (j-1)*nx + i,
+ * negated on the z = 0 face, 0 unless detected), tdet (time at the end [ns]) and
+ * bnc (bounces).  Any of the four output pointers may be NULL. */
+int crysp_transport_records(crysp_ctx *ctx, const crysp_box *box,
+                            const crysp_optical_params *op, const crysp_readout *ro,
+                            const float *px, const float *py, const float *pz,
+                            const float *pt, int32_t npos, int64_t n_photons,
+                            uint64_t seed, uint64_t pid0, int32_t max_bounces,
+                            const crysp_emission *em,
+                            uint8_t *status, int16_t *idx, float *tdet, int32_t *bnc);
+
+/* Accumulate mode: the same photons reduced on the device into the reference's
+ * Accumulator (run_photons_ka! + its host loop, src/kernel.jl:312-367, and
+ * run_photons!, src/transport.jl:245-273).  counts: u32 (nx, ny, nbins+1);
+ * first_ns: f32 (nx, ny), +Inf where nothing was detected; counters: int64
+ * [CRYSP_NCOUNTERS] (crysp_counter).  Outputs are overwritten, not added to. */
+int crysp_transport_accumulate(crysp_ctx *ctx, const crysp_box *box,
+                               const crysp_optical_params *op, const crysp_readout *ro,
+                               const crysp_time_binning *tb,
+                               const float *px, const float *py, const float *pz,
+                               const float *pt, int32_t npos, int64_t n_photons,
+                               uint64_t seed, uint64_t pid0, int32_t max_bounces,
+                               const crysp_emission *em,
+                               uint32_t *counts, float *first_ns, int64_t *counters);
+
+/* Event mode (replaces the batch loop of run_events!, src/events.jl:59-91,109-121):
+ * ndep deposits, deposit d holding dep_xyzt[4d..4d+3] = (x, y, z [mm], t0 [ns]),
+ * dep_nph[d] photons and belonging to event dep_event[d] (0-based, < nev).  The photons
+ * of all deposits are numbered g = 0.. in deposit order (deposit-major, as the
+ * reference pushes them) and photon g uses stream pid = pid0 + g + 1; one persistent
+ * launch drains them through a device work queue.  Outputs (overwritten): maps i32
+ * (nx, ny, nev), tmin f32 (nx, ny, nev) (+Inf = no hit), npe i32 [nev]; both readout
+ * faces count (src/events.jl:78).  n_photons_out (optional) = total photons. */
+int crysp_run_events(crysp_ctx *ctx, const crysp_box *box,
+                     const crysp_optical_params *op, const crysp_readout *ro,
+                     const float *dep_xyzt, const int32_t *dep_nph,
+                     const int32_t *dep_event, int64_t ndep, int64_t nev,
+                     uint64_t seed, uint64_t pid0, int32_t max_bounces,
+                     const crysp_emission *em,
+                     int32_t *maps, float *tmin, int32_t *npe, int64_t *n_photons_out);
+
+/* Launch schedule of the transport kernels: workgroup size (64..1024, multiple of 64)
+ * and photons per thread (1..8; sets the persistent grid to
+ * ceil(n_photons / (wg * ppt)) workgroups, capped at what the device keeps resident).
+ * No reference counterpart (KA picks the workgroup, src/kernel.jl:8). */
+int crysp_set_schedule(crysp_ctx *ctx, int32_t wg_size, int32_t photons_per_thread);
+int crysp_get_schedule(crysp_ctx *ctx, int32_t *wg_size, int32_t *photons_per_thread);
+
+/* Autotuner (no reference counterpart): times crysp_run_events on the given deposits
+ * for every (wg_sizes[a], ppts[b]) with `reps` repetitions each, keeps the fastest as
+ * the context's schedule and writes the best-of-reps kernel time [ms] of every pair
+ * into times_ms[a * n_ppt + b].  Results do not depend on the schedule. */
+int crysp_autotune_events(crysp_ctx *ctx, const crysp_box *box,
+                          const crysp_optical_params *op, const crysp_readout *ro,
+                          const float *dep_xyzt, const int32_t *dep_nph,
+                          const int32_t *dep_event, int64_t ndep, int64_t nev,
+                          uint64_t seed, int32_t max_bounces, const crysp_emission *em,
+                          const int32_t *wg_sizes, int32_t n_wg,
+                          const int32_t *ppts, int32_t n_ppt, int32_t reps,
+                          float *times_ms, int32_t *best_wg, int32_t *best_ppt);
+
+/* Debug: the first n_draws raw u32 words of the streams pid0+1 .. pid0+n_pids,
+ * produced on the device by the same Philox code the kernels use
+ * (src/philox.jl:43-57).  out: n_pids * n_draws words, pid-major. */
+int crysp_debug_draws(crysp_ctx *ctx, uint64_t seed, uint64_t pid0, int64_t n_pids,
+                      int32_t n_draws, uint32_t *out);
+
+/* Debug: the device Float32 functions the transport uses, applied to n inputs, for
+ * the exhaustive checks of their domains.  which: 0 log, 1 sin, 2 cos (both from the
+ * sincos path), 3 sqrt, 4 cos (the stand-alone cos of the Box-Muller gaussian). */
+int crysp_debug_math(crysp_ctx *ctx, int32_t which, const float *x, int64_t n, float *y);
+
+/* Kernel timing: when enabled, every transport launch is bracketed by hipEvents on
+ * the context's stream; crysp_kernel_time returns the summed kernel time [ms] and
+ * the number of launches since the last reset. */
+int crysp_set_timing(crysp_ctx *ctx, int32_t enable);
+int crysp_kernel_time(crysp_ctx *ctx, double *total_ms, int64_t *n_launches, int32_t reset);
+
+/* Operation counting: when enabled, transport launches use a kernel variant that also
+ * counts, per call, [n_photon, n_draw, n_philox_block, n_step, n_fresnel,
+ * n_surface_try, n_lambert, n_rayleigh_try, n_detect, n_scatter]; crysp_get_op_counts
+ * returns their sums since the last reset (the inputs of the roofline op model). */
+int crysp_set_op_counting(crysp_ctx *ctx, int32_t enable);
+int crysp_get_op_counts(crysp_ctx *ctx, int64_t *counts10, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRYSPLIGHT_H */

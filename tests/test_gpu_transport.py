@@ -1,0 +1,128 @@
+"""GPU parity of the HIP transport against the CPU oracle, through the C-ABI.
+
+Bar: integer outputs bit-exact; per-photon fates identical on >= 99.9% of photons
+(in practice every photon: the transcendentals are correctly rounded on both sides)."""
+import numpy as np
+import pytest
+
+import oracle
+from crysplight_amd import transport as T
+from crysplight_amd.params import Box, SipmGrid, TimeBinning, readout, deg2rad
+from tests import cases as K
+
+pytestmark = pytest.mark.gpu
+
+
+def _fate_match(gpu, ref):
+    same = ((gpu.status == ref["status"]) & (gpu.idx == ref["idx"])
+            & (gpu.bounces == ref["bounces"]))
+    return same.mean(), same
+
+
+def test_philox_streams_bit_exact(ctx):
+    for seed, pid0 in [(1234, 0), (42, 6), (2**40 + 7, 2**32 + 3)]:
+        g = T.debug_draws(seed, 300, 64, pid0=pid0, ctx=ctx)
+        r = oracle.debug_draws(seed, 300, 64, pid0=pid0)
+        assert np.array_equal(g, r), (seed, pid0)
+
+
+def test_log_exhaustive_on_uniform_domain(ctx):
+    """log is only ever evaluated at randu outputs k * 2^-24, k = 1..2^24: check all."""
+    u = (np.arange(1, 2**24 + 1, dtype=np.float64) * 2.0**-24).astype(np.float32)
+    g = T.debug_math("log", u, ctx=ctx)
+    ref = np.log(u.astype(np.float64)).astype(np.float32)
+    bad = np.flatnonzero(g != ref)
+    assert bad.size == 0, (bad[:10], u[bad[:10]])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos", "cos1"])
+def test_sincos_exhaustive_on_phase_domain(ctx, fn):
+    """sin/cos of phi = 2f0*Float32(pi)*u for every randu value u."""
+    u = (np.arange(1, 2**24 + 1, dtype=np.float64) * 2.0**-24).astype(np.float32)
+    phi = np.float32(6.2831855) * u
+    g = T.debug_math(fn, phi, ctx=ctx)
+    ref = getattr(np, fn[:3])(phi.astype(np.float64)).astype(np.float32)
+    bad = np.flatnonzero(g != ref)
+    assert bad.size == 0, (bad[:10], phi[bad[:10]])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_sincos_on_tilt_domain(ctx, fn):
+    """alpha = sigma*|g| < 1.2: every 7th float in [0, 1.2)."""
+    lo = np.float32(0).view(np.uint32)
+    hi = np.float32(1.2).view(np.uint32)
+    x = np.arange(lo, hi, 7, dtype=np.uint32).view(np.float32)
+    g = T.debug_math(fn, x, ctx=ctx)
+    ref = getattr(np, fn)(x.astype(np.float64)).astype(np.float32)
+    assert np.count_nonzero(g != ref) == 0
+
+
+def test_sqrt_correctly_rounded(ctx):
+    """sqrt is evaluated on [0, 4): every 3rd float there, plus all of [0.25, 1)."""
+    hi = np.float32(4.0).view(np.uint32)
+    x = np.concatenate([np.arange(0, hi, 3, dtype=np.uint32).view(np.float32),
+                        np.arange(np.float32(0.25).view(np.uint32),
+                                  np.float32(1.0).view(np.uint32), dtype=np.uint32).view(np.float32)])
+    g = T.debug_math("sqrt", x, ctx=ctx)
+    assert np.array_equal(g, np.sqrt(x))
+
+
+def _records_case(ctx, box, op, ro, n, seed, pos, tau, maxb=100_000, positions=None,
+                  t0s=None, pid0=0, tau_rise=0.0):
+    kw = dict(n_photons=n, seed=seed, pos=pos, positions=positions, t0s=t0s,
+              pid_offset=pid0, tau_ns=tau, tau_rise_ns=tau_rise, max_bounces=maxb)
+    g = T.transport_records(box, op, ro, ctx=ctx, **kw)
+    r = oracle.transport_records(box, op, ro, **kw)
+    frac, same = _fate_match(g, r)
+    tdiff = np.flatnonzero(same & (g.t != r["t"]))
+    print(f"fate parity {frac:.6f} ({(~same).sum()} differ), time mismatches {tdiff.size}")
+    assert frac >= 0.999, frac
+    assert tdiff.size == 0, tdiff[:10]
+    return g, r, frac
+
+
+def test_records_c1_point_source(ctx):
+    g, r, frac = _records_case(ctx, K.C1_BOX, K.C1_OP, K.C1_RO, 100_000, K.C1_SEED,
+                               K.C1_POS, K.C1_TAU)
+    assert frac == 1.0
+
+
+def test_records_all_processes(ctx):
+    g, r, frac = _records_case(ctx, K.BOX_CRYSP, K.OP_ALL, K.RO_ALL, 100_000, 8,
+                               K.POS_CRYSP, 800.0)
+    assert frac == 1.0
+
+
+def test_records_two_sided_rise_multi_position(ctx):
+    ro2 = readout(K.GRID, two_sided=True)
+    pos = [(24.0, 24.0, 5.0), (24.0, 24.0, 30.0), (3.0, 40.0, 20.0)]
+    g, r, frac = _records_case(ctx, K.BOX_CRYSP, K.OP_ALL, ro2, 60_000, 23, None, 800.0,
+                               positions=pos, t0s=[0.0, 5.0, 1e6], pid0=1_000_003,
+                               tau_rise=20.0)
+    assert frac == 1.0
+    assert (g.idx < 0).any() and (g.idx > 0).any()
+
+
+def test_accumulate_equals_reduced_records(ctx):
+    kw = dict(n_photons=200_000, seed=8, pos=K.POS_CRYSP, tau_ns=800.0)
+    acc = T.transport_accumulate(K.BOX_CRYSP, K.OP_ALL, K.RO_ALL, K.TB, ctx=ctx, **kw)
+    r = oracle.run_photons(K.BOX_CRYSP, K.OP_ALL, K.RO_ALL, K.TB, **kw)
+    assert np.array_equal(acc.counts, r["counts"])
+    assert np.array_equal(acc.first_ns, r["first_ns"])
+    for name in ("ndet", "ndet_front", "nabs_bulk", "nabs_wall", "nabs_sipm", "ncap",
+                 "nabs_sur", "nscat", "sum_bounces_det"):
+        assert getattr(acc, name) == r[name], name
+    assert acc.total_terminated() == 200_000
+
+
+def test_op_counts_match_oracle(ctx):
+    kw = dict(n_photons=50_000, seed=1234, pos=K.C1_POS, tau_ns=K.C1_TAU)
+    ctx.set_op_counting(True)
+    try:
+        ctx.op_counts(reset=True)
+        T.transport_records(K.C1_BOX, K.C1_OP, K.C1_RO, ctx=ctx, **kw)
+        ops = ctx.op_counts(reset=True)
+    finally:
+        ctx.set_op_counting(False)
+    r = oracle.transport_records(K.C1_BOX, K.C1_OP, K.C1_RO, count_ops=True, **kw)
+    assert list(ops) == [r["ops"][n] for n in oracle.OPCOUNT_NAMES]
