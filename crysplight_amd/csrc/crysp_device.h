@@ -40,6 +40,7 @@ struct KParams {
     uint64_t pid0;
     int64_t n_photons;
     int64_t ndep;
+    int64_t nev;
     int32_t chunk;                            // photons per work-queue grab
     int32_t hist_words;                       // nx*ny*(nbins+1)
 };

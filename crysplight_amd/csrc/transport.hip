@@ -43,6 +43,7 @@ struct KBuffers {
     int32_t *maps; uint32_t *ev_tkeys;              // events
     unsigned long long *cursor;                     // work-queue cursor (zeroed)
     unsigned long long *opcounts;                   // 10 op counters (COUNT only)
+    unsigned int *flag;                             // events: input error flags
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -105,7 +106,9 @@ transport_kernel(const KParams P, const KBuffers B)
 
     uint64_t pool_next = 0, pool_end = 0;           // wave-uniform work pool
     bool exhausted = false;
-    const uint64_t N = (uint64_t)P.n_photons;
+    // event mode reads the photon total the offsets scan produced
+    const uint64_t N = (MODE == M_EVENTS && P.n_photons < 0) ? (uint64_t)B.dep_off[P.ndep]
+                                                             : (uint64_t)P.n_photons;
 
     for (;;) {
         // ---- refill lanes whose photon ended --------------------------------------
@@ -190,7 +193,7 @@ transport_kernel(const KParams P, const KBuffers B)
                     else if (st == ST_ABS_SUR) ++c_sur;
                     else ++c_cap;
                 } else {
-                    if (st == ST_DETECTED) {
+                    if (st == ST_DETECTED && (uint32_t)ev < (uint32_t)P.nev) {
                         const int l = lin < 0 ? -lin : lin;
                         const int64_t cell = (int64_t)(l - 1) + (int64_t)nxy * ev;
                         atomicAdd(&B.maps[cell], 1);
@@ -254,6 +257,40 @@ __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *
     }
 }
 
+// exclusive scan of the per-deposit photon counts (device-resident inputs); one
+// workgroup, each thread a contiguous slice.  off[ndep] = total.  flag bit 1: a negative
+// count (taken as 0), bit 2: an event index outside [0, nev).
+__global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, const int32_t *ev,
+                                                      int64_t ndep, int64_t nev, int64_t *off,
+                                                      unsigned int *flag)
+{
+    __shared__ int64_t part[1024];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int64_t per = (ndep + nt - 1) / nt;
+    const int64_t lo = std::min<int64_t>(ndep, (int64_t)tid * per);
+    const int64_t hi = std::min<int64_t>(ndep, lo + per);
+    int64_t s = 0;
+    unsigned bad = 0u;
+    for (int64_t i = lo; i < hi; ++i) {
+        const int32_t v = nph[i];
+        if (v < 0) bad |= 1u;
+        s += v > 0 ? v : 0;
+        const int32_t e = ev[i];
+        if (e < 0 || (int64_t)e >= nev) bad |= 2u;
+    }
+    if (bad) atomicOr(flag, bad);
+    part[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t acc = 0;
+        for (int i = 0; i < nt; ++i) { const int64_t v = part[i]; part[i] = acc; acc += v; }
+        off[ndep] = acc;
+    }
+    __syncthreads();
+    int64_t acc = part[tid];
+    for (int64_t i = lo; i < hi; ++i) { off[i] = acc; const int32_t v = nph[i]; acc += v > 0 ? v : 0; }
+}
+
 __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids, int n_draws,
                                    uint32_t *out)
 {
@@ -299,6 +336,8 @@ static int fail(int code, const std::string &msg)
             return fail(CRYSP_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+constexpr int B_NBUF_MAX = 32;
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -319,7 +358,8 @@ struct crysp_ctx {
     bool count_ops = false;
     unsigned long long ops_host[10] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf bufs[24];
+    DevBuf bufs[B_NBUF_MAX];
+    int64_t last_n_hint = 0;
     // captured event-mode graph
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_key;
@@ -327,7 +367,7 @@ struct crysp_ctx {
 
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
-             B_NPE, B_OPS, B_MISC, B_MISC2, B_NBUF };
+             B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_NBUF };
 
 static int ensure(crysp_ctx *c, int id, size_t bytes, void **out)
 {
@@ -683,8 +723,11 @@ int crysp_transport_accumulate(crysp_ctx *c, const crysp_box *box,
     return CRYSP_OK;
 }
 
-// Event mode with the launch sequence captured once in a hipGraph and replayed while
-// the arguments (pointers, sizes, parameters) stay the same.
+// Event mode.  Deposit photon offsets come from the host (H2D) when the counts live in
+// host memory, or from a device scan when they live in device memory; the transport
+// kernel reads the photon total from dep_off[ndep], so nothing forces a host round trip
+// and the whole launch sequence is captured once in a hipGraph and replayed while the
+// arguments (pointers, sizes, parameters, schedule) stay the same.
 static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
                        const crysp_readout *ro, const float *dep_xyzt, const int32_t *dep_nph,
                        const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
@@ -696,46 +739,42 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     if (ndep > 0 && (!dep_xyzt || !dep_nph || !dep_event))
         return fail(CRYSP_EINVAL, "null deposit arrays");
     if (nev > 0 && (!maps || !tmin || !npe)) return fail(CRYSP_EINVAL, "null outputs");
+    if (nev > INT32_MAX) return fail(CRYSP_ERANGE, "more than 2^31-1 events");
     KParams P;
     int rc = fill_params(P, box, op, ro, em, seed, pid0, maxb);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
     const int nxy = ro->grid.nx * ro->grid.ny;
-    // deposit table: photon offsets are computed on the host when the counts live there
-    int64_t ntot = 0;
-    const bool nph_dev = is_device_ptr(dep_nph);
-    const bool ev_dev = is_device_ptr(dep_event);
-    std::vector<int32_t> hnph, hev;
-    const int32_t *nph_h = dep_nph, *ev_h = dep_event;
-    if (nph_dev && ndep) {
-        hnph.resize(ndep);
-        HIPCHK(hipMemcpy(hnph.data(), dep_nph, ndep * 4, hipMemcpyDeviceToHost));
-        nph_h = hnph.data();
-    }
-    if (ev_dev && ndep) {
-        hev.resize(ndep);
-        HIPCHK(hipMemcpy(hev.data(), dep_event, ndep * 4, hipMemcpyDeviceToHost));
-        ev_h = hev.data();
-    }
-    std::vector<int64_t> off(ndep + 1, 0);
-    for (int64_t d = 0; d < ndep; ++d) {
-        if (nph_h[d] < 0) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
-        if (ev_h[d] < 0 || ev_h[d] >= nev) return fail(CRYSP_EINVAL, "dep_event out of range");
-        off[d + 1] = off[d] + nph_h[d];
-    }
-    ntot = off[ndep];
-    if (n_out) *n_out = ntot;
-    P.n_photons = ntot;
+    const bool nph_dev = ndep > 0 && is_device_ptr(dep_nph);
     P.ndep = ndep;
+    P.nev = nev;
     KBuffers B{};
-    void *doff;
-    if ((rc = ensure(c, B_DEPOFF, (size_t)(ndep + 1) * 8, &doff))) return rc;
-    HIPCHK(hipMemcpyAsync(doff, off.data(), (size_t)(ndep + 1) * 8, hipMemcpyHostToDevice,
-                          c->stream));
+    void *doff, *dflag;
+    if ((rc = ensure(c, B_DEPOFF, (size_t)(ndep + 1) * 8, &doff)) ||
+        (rc = ensure(c, B_FLAG, 16, &dflag)))
+        return rc;
     B.dep_off = (const int64_t *)doff;
+    B.flag = (unsigned int *)dflag;
+    int64_t ntot_host = -1;
+    std::vector<int64_t> off;                       // alive until the final sync
+    if (!nph_dev) {
+        off.assign(ndep + 1, 0);
+        for (int64_t d = 0; d < ndep; ++d) {
+            if (dep_nph[d] < 0) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
+            off[d + 1] = off[d] + dep_nph[d];
+        }
+        if (!is_device_ptr(dep_event))
+            for (int64_t d = 0; d < ndep; ++d)
+                if (dep_event[d] < 0 || dep_event[d] >= nev)
+                    return fail(CRYSP_EINVAL, "dep_event out of range");
+        ntot_host = off[ndep];
+        HIPCHK(hipMemcpyAsync(doff, off.data(), (size_t)(ndep + 1) * 8, hipMemcpyHostToDevice,
+                              c->stream));
+    }
     if ((rc = stage_in(c, B_DEP, dep_xyzt, (size_t)ndep * 16, (const void **)&B.dep)) ||
         (rc = stage_in(c, B_DEPEV, dep_event, (size_t)ndep * 4, (const void **)&B.dep_event)))
         return rc;
+    const int32_t *nph_d = nph_dev ? dep_nph : nullptr;
     void *dmaps, *dtmin, *dnpe, *dkeys;
     const size_t cells = (size_t)nev * nxy;
     if ((rc = out_buf(c, B_MAPS, maps, cells * 4, &dmaps)) ||
@@ -748,74 +787,96 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     void *cur;
     if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
     B.cursor = (unsigned long long *)cur;
+    if (c->count_ops) {
+        void *ops;
+        if ((rc = ensure(c, B_OPS, 80, &ops))) return rc;
+        B.opcounts = (unsigned long long *)ops;
+    }
     P.chunk = c->chunk;
-    const int grid = grid_for(c, ntot, c->wg, c->ppt, 0);
+    // the grid only sizes the persistent launch; the kernel drains whatever total it reads
+    const int64_t n_hint = ntot_host >= 0 ? ntot_host : (c->last_n_hint > 0 ? c->last_n_hint : (int64_t)1 << 40);
+    P.n_photons = -1;                               // read dep_off[ndep] on the device
+    const int grid = grid_for(c, n_hint, c->wg, c->ppt, 0);
 
-    // graph key: everything the captured launches depend on
-    std::vector<unsigned char> key(sizeof P + sizeof B + 4 * sizeof(int64_t));
+    std::vector<unsigned char> key(sizeof P + sizeof B + sizeof(void *) + 6 * sizeof(int64_t));
     {
-        int64_t extra[4] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops };
+        int64_t extra[6] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
+                             (int64_t)c->timing, (int64_t)nph_dev };
         unsigned char *q = key.data();
         std::memcpy(q, &P, sizeof P); q += sizeof P;
         std::memcpy(q, &B, sizeof B); q += sizeof B;
+        std::memcpy(q, &nph_d, sizeof(void *)); q += sizeof(void *);
         std::memcpy(q, extra, sizeof extra);
     }
-    const bool graphable = use_graph && !c->count_ops && !c->timing;
-    if (graphable && c->graph && key == c->graph_key) {
-        HIPCHK(hipGraphLaunch(c->graph, c->stream));
-    } else {
-        auto enqueue = [&]() -> int {
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
-            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
-                               (int64_t)cells);
-            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
-                               KEY_INF, (int64_t)cells);
-            if (ntot > 0) {
-                if (c->count_ops) {
-                    void *ops;
-                    int r2 = ensure(c, B_OPS, 80, &ops);
-                    if (r2) return r2;
-                    B.opcounts = (unsigned long long *)ops;
-                    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)ops, 0u, 20);
-                }
-                if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
-                if (c->count_ops)
-                    hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg),
-                                       0, c->stream, P, B);
-                else
-                    hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg),
-                                       0, c->stream, P, B);
-                if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
-            }
-            if (nev > 0)
-                hipLaunchKernelGGL(finish_events, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
-                                   dim3(256), 0, c->stream, (const int32_t *)dmaps,
-                                   (const uint32_t *)dkeys, (float *)dtmin, (int32_t *)dnpe, nev, nxy);
-            return CRYSP_OK;
-        };
-        if (graphable) {
-            if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-            hipGraph_t g;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue();
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(CRYSP_EHIP, "hipStreamEndCapture failed");
-            HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-            c->graph_key = key;
+    auto enqueue = [&]() -> int {
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)dflag, 0u, 4);
+        if (nph_d)
+            hipLaunchKernelGGL(scan_deposits, dim3(1), dim3(1024), 0, c->stream, nph_d,
+                               B.dep_event, ndep, nev, (int64_t *)doff, (unsigned int *)dflag);
+        hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
+                           (int64_t)cells);
+        hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
+                           KEY_INF, (int64_t)cells);
+        if (c->count_ops)
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
+        if (ndep > 0) {
+            if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
+            if (c->count_ops)
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), 0,
+                                   c->stream, P, B);
+            else
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg), 0,
+                                   c->stream, P, B);
+            if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
+        }
+        if (nev > 0)
+            hipLaunchKernelGGL(finish_events, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
+                               dim3(256), 0, c->stream, (const int32_t *)dmaps, (const uint32_t *)dkeys,
+                               (float *)dtmin, (int32_t *)dnpe, nev, nxy);
+        return CRYSP_OK;
+    };
+    bool launched = false;
+    if (use_graph) {
+        if (c->graph && key == c->graph_key) {
             HIPCHK(hipGraphLaunch(c->graph, c->stream));
+            launched = true;
         } else {
-            if ((rc = enqueue())) return rc;
-            HIPCHK(hipGetLastError());
+            if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+            c->graph_key.clear();
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                rc = enqueue();
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+                if (e == hipSuccess && g &&
+                    hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0) == hipSuccess) {
+                    c->graph_key = key;
+                    HIPCHK(hipGraphLaunch(c->graph, c->stream));
+                    launched = true;
+                }
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (!launched) (void)hipGetLastError();   // capture unsupported here: run eagerly
         }
     }
+    if (!launched) {
+        if ((rc = enqueue())) return rc;
+        HIPCHK(hipGetLastError());
+    }
+    int64_t tail[2] = { 0, 0 };                      // photon total, error flags
+    HIPCHK(hipMemcpyAsync(&tail[0], (int64_t *)doff + ndep, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&tail[1], dflag, 4, hipMemcpyDeviceToHost, c->stream));
     if ((rc = stage_out(c, maps, dmaps, cells * 4)) ||
         (rc = stage_out(c, tmin, dtmin, cells * 4)) ||
         (rc = stage_out(c, npe, dnpe, (size_t)nev * 4)))
         return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (ntot > 0 && !graphable) return finish_timing(c);
+    c->last_n_hint = tail[0];
+    if (n_out) *n_out = tail[0];
+    if (tail[1] & 1) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
+    if (tail[1] & 2) return fail(CRYSP_EINVAL, "dep_event out of range (those deposits were skipped)");
+    if (ndep > 0) return finish_timing(c);
     return CRYSP_OK;
 }
 

@@ -191,6 +191,61 @@ int crysp_kernel_time(crysp_ctx *ctx, double *total_ms, int64_t *n_launches, int
 int crysp_set_op_counting(crysp_ctx *ctx, int32_t enable);
 int crysp_get_op_counts(crysp_ctx *ctx, int64_t *counts10, int32_t reset);
 
+/* ---- host-side event feed (CPU code in the same library) ------------------------
+ * The gamma stage that produces deposits for crysp_run_events: the reference's vendored
+ * gamma core (src/gamma/*.jl) and its bridge (src/gamma_interface.jl), Float64 cm/MeV.
+ * Random numbers come from per-event Philox streams (the reference's Xoshiro cannot be
+ * reproduced), so events agree with the reference statistically, not bit for bit. */
+typedef struct crysp_gamma_material crysp_gamma_material;
+
+/* Gamma.make_material (src/gamma/gmaterials.jl:22-31) from an XCOM table file (rows of
+ * 8 numbers: E [MeV], coherent, incoherent, photoelectric, pair nuclear, pair electron,
+ * totals; mu/rho [cm^2/g]) or from such rows in memory.  NULL on failure. */
+crysp_gamma_material *crysp_gamma_material_load(const char *xcom_path, double density);
+crysp_gamma_material *crysp_gamma_material_create(const double *rows8, int32_t nrows,
+                                                  double density);
+void crysp_gamma_material_free(crysp_gamma_material *m);
+const char *crysp_gamma_last_error(void);
+
+/* Gamma.sigma_macro (src/gamma/gmaterials.jl:45-55): (Compton, photoelectric, pair)
+ * macroscopic cross sections [1/cm] at e_mev. */
+int crysp_gamma_sigma(const crysp_gamma_material *m, double e_mev, double *out3);
+
+/* Gamma.propagate_photon (src/gamma/gtransport.jl:44-81) in the box-centred frame (cm):
+ * up to cap records (x, y, z, e_in, e_dep, process) with process 0 photoelectric,
+ * 1 Compton, 2 below cut, 3 pair, 4 escape; n_out = number of records. */
+int crysp_gamma_history(const crysp_gamma_material *m, const crysp_box *box, double e0_mev,
+                        const double *p0_cm, const double *dir0, uint64_t seed,
+                        uint64_t stream, double *recs6, int32_t cap, int32_t *n_out);
+
+/* The event loop of run_events! (src/events.jl:93-117): per event a gamma of e0_mev
+ * enters the z = 0 face at a uniform point along +z; its deposits (crystal frame:
+ * x, y, z [mm], t [ns]) get Poisson(yield_per_mev * E / 1000) photons.  Deposits are
+ * written densely (event e owns [ev_first[e], ev_first[e+1])); dep_* arrays need
+ * n_events * max_dep_per_event entries.  Truth (optional pointers): edep, xyz1, e1,
+ * xyz2, e2, er [keV, mm], int_type (-1 none, 0 photoelectric, X Compton scatters),
+ * n_int, as in src/events.jl:25-37. */
+int crysp_gamma_events(const crysp_gamma_material *m, const crysp_box *box, int64_t n_events,
+                       uint64_t seed, double e0_mev, double yield_per_mev,
+                       int32_t max_dep_per_event, float *dep_xyzt, int32_t *dep_nph,
+                       int32_t *dep_event, float *dep_e_kev, int8_t *dep_proc,
+                       int64_t *ev_first, float *edep, float *xyz1, float *e1, float *xyz2,
+                       float *e2, float *er, int8_t *int_type, int16_t *n_int,
+                       int64_t *n_dep_out);
+
+/* Synthetic single-deposit events (no reference counterpart; the batched and surrogate
+ * workloads): position uniform in the axis-aligned box [lo3, hi3] mm, energy uniform
+ * in [e_lo, e_hi] keV, t0 = 0, photons Poisson(yield * E / 1000), or n_fixed each when
+ * yield_per_mev <= 0. */
+int crysp_synthetic_deposits(int64_t n_events, uint64_t seed, const float *lo3,
+                             const float *hi3, float e_lo_kev, float e_hi_kev,
+                             double yield_per_mev, int32_t n_fixed, float *dep_xyzt,
+                             float *dep_e_kev, int32_t *dep_nph);
+
+/* rand_poisson (src/events.jl:11-22: Knuth for lambda <= 50, rounded Gaussian above)
+ * on n independent Philox streams, for the sampler's tests. */
+int crysp_poisson_samples(uint64_t seed, double lambda, int64_t n, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

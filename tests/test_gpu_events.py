@@ -1,0 +1,106 @@
+"""GPU parity of the event mode (work queue, device maps, hipGraph replay, autotuner)
+against the oracle's per-event reduction (src/events.jl:73-88)."""
+import numpy as np
+import pytest
+
+import oracle
+from crysplight_amd import events as E
+from crysplight_amd.params import Box, SipmGrid, readout
+from tests import cases as K
+
+pytestmark = pytest.mark.gpu
+
+RO8 = readout(SipmGrid(3.2, 8, 8))
+
+
+def _oracle_maps(box, op, ro, dep, seed, tau, tau_rise=0.0, pid0=0):
+    return oracle.run_events(box, op, ro, dep.xyzt, dep.nph, dep.event, dep.n_events,
+                             seed=seed, pid0=pid0, tau_ns=tau, tau_rise_ns=tau_rise)
+
+
+def _check(res, ref, nev):
+    assert np.array_equal(res.maps.reshape(nev, -1), ref["maps"])
+    assert np.array_equal(res.npe, ref["npe"])
+    assert np.array_equal(res.tmin.reshape(nev, -1), ref["tmin"])
+
+
+@pytest.fixture(scope="module")
+def bgo_events():
+    bgo = E.GammaMaterial("xcom_BGO.csv", 7.13)
+    dep, truth = E.gamma_events(K.C2_BOX, bgo, 8500.0, n_events=400, seed=7)
+    return dep, truth
+
+
+def test_events_bgo_host_arrays(ctx, bgo_events):
+    dep, _ = bgo_events
+    res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+    assert res.n_photons == dep.n_photons
+    _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU), dep.n_events)
+
+
+def test_events_device_arrays_graph_replays(ctx, bgo_events):
+    """Device-resident inputs go through the captured graph; replays with changed
+    contents (same pointers) must track the contents, with eager work in between."""
+    import torch
+    dep, _ = bgo_events
+    dev = torch.device("cuda", 0)
+    xyzt = torch.from_numpy(dep.xyzt).to(dev)
+    nph = torch.from_numpy(dep.nph).to(dev)
+    ev = torch.from_numpy(dep.event).to(dev)
+    nev = dep.n_events
+    out = (torch.zeros((nev, 8, 8), dtype=torch.int32, device=dev),
+           torch.zeros((nev, 8, 8), dtype=torch.float32, device=dev),
+           torch.zeros(nev, dtype=torch.int32, device=dev))
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    for rep in range(3):
+        r = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                               device_arrays=(xyzt, nph, ev), out=out)
+        assert r.n_photons == dep.n_photons
+        assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref["maps"]), rep
+        assert np.array_equal(out[1].cpu().numpy().reshape(nev, -1), ref["tmin"]), rep
+        assert np.array_equal(out[2].cpu().numpy(), ref["npe"]), rep
+        _ = (xyzt * 2).sum().item()                 # eager work between replays
+    # same pointers, new contents: half the photons
+    half = E.Deposits(dep.xyzt, dep.nph // 2, dep.event, dep.e_kev, nev)
+    nph.copy_(torch.from_numpy(half.nph))
+    r = E.transport_events(K.C2_BOX, K.C2_OP, RO8, half, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=(xyzt, nph, ev), out=out)
+    ref2 = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, half, 7, K.C2_TAU)
+    assert r.n_photons == half.n_photons
+    assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref2["maps"])
+
+
+def test_events_rise_time_uneven(ctx):
+    """Batched uneven single-deposit events with rise + decay emission."""
+    dep = E.synthetic_deposits(K.C1_BOX, n_events=300, seed=11, e_kev=(10.0, 511.0),
+                               yield_per_mev=2000.0)
+    assert dep.nph.min() < 100 < 900 < dep.nph.max()
+    res = E.transport_events(K.C1_BOX, K.C1_OP, RO8, dep, seed=5, tau_ns=1000.0,
+                             tau_rise_ns=20.0, pid0=12345, ctx=ctx)
+    _check(res, _oracle_maps(K.C1_BOX, K.C1_OP, RO8, dep, 5, 1000.0, 20.0, 12345), 300)
+
+
+def test_events_schedule_independent_and_autotune(ctx, bgo_events):
+    dep, _ = bgo_events
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    old = ctx.get_schedule()
+    try:
+        for wg, ppt in [(64, 1), (1024, 8), (192, 3)]:
+            ctx.set_schedule(wg, ppt)
+            res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+            _check(res, ref, dep.n_events)
+        best, times = E.autotune(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                                 wg_sizes=(64, 256, 1024), ppts=(1, 4), reps=1)
+        assert best in [(w, p) for w in (64, 256, 1024) for p in (1, 4)]
+        assert np.all(times > 0)
+        assert ctx.get_schedule() == best
+    finally:
+        ctx.set_schedule(*old)
+
+
+def test_events_bad_event_index_is_an_error(ctx, bgo_events):
+    dep, _ = bgo_events
+    bad = E.Deposits(dep.xyzt[:5], dep.nph[:5], np.array([0, 1, 2, 3, 99], np.int32),
+                     dep.e_kev[:5], 4)
+    with pytest.raises(Exception):
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, bad, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
