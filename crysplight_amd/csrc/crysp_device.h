@@ -5,7 +5,8 @@
 // operation gives the same bits as the Julia Float32 code it follows.  The
 // transcendentals (log, sin, cos) are evaluated in binary64 and rounded once to
 // binary32; Julia's Float32 versions are computed the same way and are correctly
-// rounded in practice.
+// rounded in practice (explicit fma() calls below are part of those binary64 kernels,
+// not contractions of the Float32 physics).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,22 +48,24 @@ struct KParams {
 
 // ------------------------------------------------------------------ Philox ----
 // Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
-__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                         uint32_t k0, uint32_t k1, uint32_t &o0,
-                                         uint32_t &o1, uint32_t &o2, uint32_t &o3)
+// Kept out of line: the stream refills at ~20 draw sites, and one shared body keeps
+// the kernel's code and scalar-register pressure small.
+__device__ __noinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        c0 = hi1 ^ c1 ^ k0;
-        c1 = lo1;
-        c2 = hi0 ^ c3 ^ k1;
-        c3 = lo0;
+        // one 32x32->64 product each (v_mad_u64_u32) gives both halves
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        c0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        c1 = (uint32_t)p1;
+        c2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
-    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+    return make_uint4(c0, c1, c2, c3);
 }
 
 // Per-photon stream: counter (pid lo, pid hi, block, 0); four words per block, used
@@ -90,7 +93,8 @@ template <bool COUNT>
 __device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<COUNT> &oc)
 {
     if (r.left == 0u) {
-        philox10(r.lo, r.hi, r.blk, 0u, P.key0, P.key1, r.w0, r.w1, r.w2, r.w3);
+        const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, P.key0, P.key1);
+        r.w0 = b.x; r.w1 = b.y; r.w2 = b.z; r.w3 = b.w;
         r.blk += 1u;
         r.left = 4u;
         oc.add(2);
@@ -115,15 +119,86 @@ __device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> 
 }
 
 // ----------------------------------------------------------- transcendentals ----
-__device__ __forceinline__ float f_log(float x) { return (float)::log((double)x); }
-__device__ __forceinline__ void f_sincos(float x, float &s, float &c)
+// The transport evaluates log only at randu outputs (0, 1] and sin/cos only at
+// phi = 2f0*Float32(pi)*u in (0, 2pi] and at tilts alpha in [0, 1.2).  These versions
+// are written for that domain (any |x| < 2^20 for sin/cos, any positive normal float
+// for log): the float argument is widened to binary64, reduced exactly, expanded in a
+// short series with ~1 ulp binary64 error, and rounded once to binary32 -- the
+// correctly rounded result except when the exact value lies within ~2^-52 relative of
+// a binary32 rounding boundary.  The tests compare every one of the 2^24 log and
+// phase arguments against a correctly rounded reference.
+
+// log(x) = e ln2 + 2 atanh(s),  x = 2^e m,  m in [sqrt(2)/2, sqrt(2)),  s = (m-1)/(m+1),
+// |s| <= 0.1716: the atanh series to s^23 leaves < 2^-55 relative.
+__device__ __forceinline__ float f_log(float x)
 {
-    double sd, cd;
-    ::sincos((double)x, &sd, &cd);
-    s = (float)sd;
-    c = (float)cd;
+    const uint32_t b = __float_as_uint(x);
+    int e = (int)(b >> 23) - 127;
+    uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) { mb -= 0x00800000u; ++e; }      // m >= sqrt(2): halve it
+    const double m = (double)__uint_as_float(mb);
+    const double s = (m - 1.0) / (m + 1.0);                  // m-1, m+1 exact
+    const double z = s * s;
+    double p = 0.043478260869565216;                         // 1/23
+    p = __builtin_fma(p, z, 0.047619047619047616);           // 1/21
+    p = __builtin_fma(p, z, 0.05263157894736842);            // 1/19
+    p = __builtin_fma(p, z, 0.058823529411764705);           // 1/17
+    p = __builtin_fma(p, z, 0.06666666666666667);            // 1/15
+    p = __builtin_fma(p, z, 0.07692307692307693);            // 1/13
+    p = __builtin_fma(p, z, 0.09090909090909091);            // 1/11
+    p = __builtin_fma(p, z, 0.1111111111111111);             // 1/9
+    p = __builtin_fma(p, z, 0.14285714285714285);            // 1/7
+    p = __builtin_fma(p, z, 0.2);                            // 1/5
+    p = __builtin_fma(p, z, 0.3333333333333333);             // 1/3
+    const double lm = 2.0 * __builtin_fma(s * z, p, s);      // 2 atanh(s)
+    const double ed = (double)e;
+    // ln2 in two parts: ed * LN2_HI is exact for |e| < 2^21
+    return (float)__builtin_fma(ed, 0.6931471803691238, __builtin_fma(ed, 1.9082149292705877e-10, lm));
 }
-__device__ __forceinline__ float f_cos(float x) { return (float)::cos((double)x); }
+
+// sin and cos: x = k pi/2 + r with pi/2 in three parts (33 + 33 + 53 bits, so k * P1
+// and x - k * P1 are exact), |r| <= pi/4, Taylor series to r^17 / r^16 (< 2^-58).
+__device__ __forceinline__ void f_sincos(float xf, float &sf, float &cf)
+{
+    const double x = (double)xf;
+    const double kd = __builtin_rint(x * 0.6366197723675814);
+    double r = __builtin_fma(-kd, 1.5707963267341256, x);
+    r = __builtin_fma(-kd, 6.077100506303966e-11, r);
+    r = __builtin_fma(-kd, 2.0222662487959506e-21, r);
+    const double z = r * r;
+    double ps = 2.8114572543455206e-15;                      // 1/17!
+    ps = __builtin_fma(ps, z, -7.647163731819816e-13);
+    ps = __builtin_fma(ps, z, 1.6059043836821613e-10);
+    ps = __builtin_fma(ps, z, -2.505210838544172e-08);
+    ps = __builtin_fma(ps, z, 2.7557319223985893e-06);
+    ps = __builtin_fma(ps, z, -0.0001984126984126984);
+    ps = __builtin_fma(ps, z, 0.008333333333333333);
+    ps = __builtin_fma(ps, z, -0.16666666666666666);
+    const double sn = __builtin_fma(r * z, ps, r);           // sin r
+    double pc = 4.779477332387385e-14;                       // 1/16!
+    pc = __builtin_fma(pc, z, -1.1470745597729725e-11);
+    pc = __builtin_fma(pc, z, 2.08767569878681e-09);
+    pc = __builtin_fma(pc, z, -2.755731922398589e-07);
+    pc = __builtin_fma(pc, z, 2.48015873015873e-05);
+    pc = __builtin_fma(pc, z, -0.001388888888888889);
+    pc = __builtin_fma(pc, z, 0.041666666666666664);
+    pc = __builtin_fma(pc, z, -0.5);
+    const double cs = __builtin_fma(z, pc, 1.0);             // cos r
+    const int q = (int)kd & 3;
+    double sv = (q & 1) ? cs : sn;
+    double cv = (q & 1) ? sn : cs;
+    if (q & 2) sv = -sv;
+    if ((q + 1) & 2) cv = -cv;
+    sf = (float)sv;
+    cf = (float)cv;
+}
+
+__device__ __forceinline__ float f_cos(float x)
+{
+    float s, c;
+    f_sincos(x, s, c);
+    return c;
+}
 
 // Julia Base semantics used by the reference
 __device__ __forceinline__ float jmax0(float v) { return (v != v) ? v : (v > 0.0f ? v : 0.0f); }

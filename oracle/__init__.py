@@ -270,6 +270,40 @@ def run_events(box, op, ro, dep_xyzt, dep_nph, dep_event, nev, *, seed, pid0=0,
     return out
 
 
+def math(which: str, x: np.ndarray) -> np.ndarray:
+    """The oracle's Float32 log / sin / cos ((float)f((double)x), glibc)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    f = lib().or_math
+    f.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]
+    f({"log": 0, "sin": 1, "cos": 2}[which], _p(x), x.size, _p(y))
+    return y
+
+
+def correctly_rounded(which: str, x: np.ndarray) -> np.ndarray:
+    """float32 rounding of the function evaluated in x87 extended precision (64-bit
+    significand): the correctly rounded binary32 value except within ~2^-63 relative
+    of a rounding boundary."""
+    xl = np.asarray(x, np.float32).astype(np.longdouble)
+    return getattr(np, which)(xl).astype(np.float32)
+
+
+def uniform_domain() -> np.ndarray:
+    """Every value randu can return: k * 2^-24, k = 1..2^24 (src/philox.jl:62)."""
+    return (np.arange(1, 2**24 + 1, dtype=np.float64) * 2.0**-24).astype(np.float32)
+
+
+def phase_domain() -> np.ndarray:
+    """Every azimuth the samplers form: 2f0*Float32(pi) * u, u in uniform_domain()."""
+    return np.float32(6.2831855) * uniform_domain()
+
+
+def tilt_domain(step: int = 7) -> np.ndarray:
+    """Every step-th binary32 value in [0, 1.2): the facet tilts alpha < 1.2."""
+    hi = np.float32(1.2).view(np.uint32)
+    return np.arange(0, hi, step, dtype=np.uint32).view(np.float32)
+
+
 def layout_sizes():
     s = np.zeros(6, np.int32)
     lib().or_layout(_p(s))

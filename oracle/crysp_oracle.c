@@ -605,6 +605,13 @@ void or_run_events(const or_box *box, const or_optical *op, const or_readout *ro
     free(off);
 }
 
+/* The oracle's Float32 transcendentals over an array (which: 0 log, 1 sin, 2 cos). */
+void or_math(int32_t which, const float *x, int64_t n, float *y)
+{
+    for (int64_t k = 0; k < n; ++k)
+        y[k] = which == 0 ? jlog(x[k]) : (which == 1 ? jsin(x[k]) : jcos(x[k]));
+}
+
 /* Struct sizes, for the layout check against the Julia isbits layout. */
 void or_layout(int32_t *sizes)
 {

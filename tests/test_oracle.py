@@ -67,6 +67,19 @@ def test_philox_rng_properties():
     assert abs(u.std(ddof=1) - math.sqrt(1 / 12)) < 3e-3
 
 
+@pytest.mark.parametrize("fn,domain", [("log", "uniform"), ("sin", "phase"), ("cos", "phase"),
+                                       ("sin", "tilt"), ("cos", "tilt")])
+def test_oracle_transcendentals_correctly_rounded(fn, domain):
+    """The oracle's (float)f((double)x) equals the binary32 rounding of the x87
+    extended-precision value on every log / phase argument randu can produce and on a
+    sample of the tilt domain; Julia's Float32 log/sin/cos are correctly rounded there
+    too, so this is where per-photon parity with the reference is decided."""
+    assert np.finfo(np.longdouble).nmant >= 63
+    x = {"uniform": oracle.uniform_domain, "phase": oracle.phase_domain,
+         "tilt": lambda: oracle.tilt_domain(61)}[domain]()
+    assert np.array_equal(oracle.math(fn, x), oracle.correctly_rounded(fn, x))
+
+
 # --------------------------------------------------------------- geometry -------
 def test_wall_hit_slab_distances():
     """test/runtests.jl:35-46"""
