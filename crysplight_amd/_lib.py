@@ -57,11 +57,23 @@ def header_symbols() -> list[str]:
     return sorted(set(re.findall(r"\b(crysp_[a-z0-9_]+)\s*\(", text)))
 
 
+def _share_hip_runtime() -> None:
+    """One HIP runtime per process.  torch's ROCm wheel ships its own libamdhip64
+    (SONAME libamdhip64.so.7) and loads it as "libamdhip64.so"; when torch is imported
+    first, this library's DT_NEEDED libamdhip64.so.7 binds to that copy.  Loaded the
+    other way round the process would hold two runtimes and torch could not see the GPU."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def load() -> C.CDLL:
     """Load the shared object (raises if it was not built)."""
     global _lib
     if _lib is not None:
         return _lib
+    _share_hip_runtime()
     if not LIB_PATH.exists():
         raise CryspError(f"{LIB_PATH} is missing: build it with `python -m crysplight_amd.build`"
                          " (there is no CPU fallback)")

@@ -808,7 +808,9 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         std::memcpy(q, &nph_d, sizeof(void *)); q += sizeof(void *);
         std::memcpy(q, extra, sizeof extra);
     }
-    auto enqueue = [&]() -> int {
+    // timing events bracket the transport kernel when launched eagerly, and the whole
+    // graph (a few-microsecond fill/scan/finish around the kernel) when replayed
+    auto enqueue = [&](bool capturing) -> int {
         hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
         hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)dflag, 0u, 4);
         if (nph_d)
@@ -821,14 +823,14 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         if (c->count_ops)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
         if (ndep > 0) {
-            if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
+            if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             if (c->count_ops)
                 hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), 0,
                                    c->stream, P, B);
             else
                 hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg), 0,
                                    c->stream, P, B);
-            if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
+            if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev1, c->stream));
         }
         if (nev > 0)
             hipLaunchKernelGGL(finish_events, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
@@ -839,20 +841,24 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     bool launched = false;
     if (use_graph) {
         if (c->graph && key == c->graph_key) {
+            if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             HIPCHK(hipGraphLaunch(c->graph, c->stream));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
             launched = true;
         } else {
             if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
             c->graph_key.clear();
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                rc = enqueue();
+                rc = enqueue(true);
                 hipError_t e = hipStreamEndCapture(c->stream, &g);
                 if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
                 if (e == hipSuccess && g &&
                     hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0) == hipSuccess) {
                     c->graph_key = key;
+                    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
                     HIPCHK(hipGraphLaunch(c->graph, c->stream));
+                    if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
                     launched = true;
                 }
                 if (g) (void)hipGraphDestroy(g);
@@ -861,7 +867,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         }
     }
     if (!launched) {
-        if ((rc = enqueue())) return rc;
+        if ((rc = enqueue(false))) return rc;
         HIPCHK(hipGetLastError());
     }
     int64_t tail[2] = { 0, 0 };                      // photon total, error flags

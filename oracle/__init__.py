@@ -72,6 +72,10 @@ def xlib():
     """rocRAND cross-check library (host build of rocRAND's header engine)."""
     global _xlib
     if _xlib is None:
+        try:
+            import torch  # noqa: F401  (share torch's HIP runtime, see crysplight_amd._lib)
+        except Exception:
+            pass
         so = HERE / "librocrand_xcheck.so"
         if not so.exists():
             subprocess.run(["make", "-s", "-C", str(HERE), str(so)], check=True)
