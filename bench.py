@@ -48,6 +48,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the steps, copy a buffer of this size once (a plain copy "
+                         "kernel of known traffic, to scale rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -104,6 +107,15 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     kern_ms, n_launch = ctx.kernel_time(reset=True)
     avg_kernel_s = kern_ms / max(n_launch, 1) / 1e3
+
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     maps_gpu = o_maps.cpu().numpy()
     npe_gpu = o_npe.cpu().numpy()

@@ -36,6 +36,7 @@ SIGNATURES = {
                                    _u64, _i32, _pem, _vp, _vp, _vp, _vp]),
     "crysp_set_schedule": (C.c_int, [_vp, _i32, _i32]),
     "crysp_get_schedule": (C.c_int, [_vp, _vp, _vp]),
+    "crysp_set_queue": (C.c_int, [_vp, _i32, _i32]),
     "crysp_autotune_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
                                         _i32, _pem, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
     "crysp_debug_draws": (C.c_int, [_vp, _u64, _u64, _i64, _i32, _vp]),
@@ -123,6 +124,9 @@ class Context:
     # schedule / timing -------------------------------------------------------
     def set_schedule(self, wg: int, ppt: int):
         check(load().crysp_set_schedule(self._h, wg, ppt), "crysp_set_schedule")
+
+    def set_queue(self, chunk: int, refill_min: int):
+        check(load().crysp_set_queue(self._h, chunk, refill_min), "crysp_set_queue")
 
     def get_schedule(self) -> tuple[int, int]:
         a, b = C.c_int32(), C.c_int32()

@@ -31,6 +31,7 @@ struct KParams {
     float L0, L1, L2;
     float n_crystal, n_coupling, abs_len, ray_len, wrap_R, sigma_alpha, pde;
     float inv_v;                              // n_crystal / c  [ns/mm]
+    float q_coup, q_air;                      // n_crystal / n_coupling, n_crystal / 1
     float cx, cy, r2, sur_R;
     float pitch, bin_ns;
     float tau_d, tau_r;
@@ -43,15 +44,14 @@ struct KParams {
     int64_t ndep;
     int64_t nev;
     int32_t chunk;                            // photons per work-queue grab
+    int32_t refill_min;                       // refill lanes once this many are idle
     int32_t hist_words;                       // nx*ny*(nbins+1)
 };
 
 // ------------------------------------------------------------------ Philox ----
 // Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
-// Kept out of line: the stream refills at ~20 draw sites, and one shared body keeps
-// the kernel's code and scalar-register pressure small.
-__device__ __noinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1)
+__device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                         uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -68,17 +68,30 @@ __device__ __noinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, ui
     return make_uint4(c0, c1, c2, c3);
 }
 
-// Per-photon stream: counter (pid lo, pid hi, block, 0); four words per block, used
-// in order (src/philox.jl:34-57).  The words shift down as they are consumed.
+// Out-of-line copy for the rare refill in the middle of a step.
+__device__ __noinline__ uint4 philox10_call(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t k0,
+                                            uint32_t k1)
+{
+    return philox10(c0, c1, c2, 0u, k0, k1);
+}
+
+// Per-photon stream: counter (pid lo, pid hi, block, 0); four words per block, used in
+// order (src/philox.jl:34-57).  The current block shifts down as it is consumed; a
+// look-ahead block is generated once per bounce step at a single site for all lanes
+// (rng_prefetch), so the draws inside a step rarely need Philox in a divergent branch.
 struct Rng {
-    uint32_t lo, hi, blk, left;
-    uint32_t w0, w1, w2, w3;
+    uint32_t lo, hi, blk, left;               // blk: index of the next block to generate
+    uint32_t w0, w1, w2, w3;                  // current block
+    uint32_t n0, n1, n2, n3;                  // look-ahead block, valid when has_next
+    uint32_t has_next;
 };
 
 __device__ __forceinline__ void rng_init(Rng &r, uint64_t pid)
 {
     r.lo = (uint32_t)pid; r.hi = (uint32_t)(pid >> 32); r.blk = 0u; r.left = 0u;
     r.w0 = r.w1 = r.w2 = r.w3 = 0u;
+    r.n0 = r.n1 = r.n2 = r.n3 = 0u;
+    r.has_next = 0u;
 }
 
 template <bool COUNT>
@@ -89,15 +102,30 @@ struct OpCount {
     __device__ __forceinline__ void add(int k, uint32_t d = 1u) { if (COUNT) v[k] += d; }
 };
 
+__device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1)
+{
+    if (!r.has_next) {
+        const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, k0, k1);
+        r.n0 = b.x; r.n1 = b.y; r.n2 = b.z; r.n3 = b.w;
+        r.blk += 1u;
+        r.has_next = 1u;
+    }
+}
+
 template <bool COUNT>
 __device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<COUNT> &oc)
 {
     if (r.left == 0u) {
-        const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, P.key0, P.key1);
-        r.w0 = b.x; r.w1 = b.y; r.w2 = b.z; r.w3 = b.w;
-        r.blk += 1u;
+        if (r.has_next) {
+            r.w0 = r.n0; r.w1 = r.n1; r.w2 = r.n2; r.w3 = r.n3;
+            r.has_next = 0u;
+        } else {
+            const uint4 b = philox10_call(r.lo, r.hi, r.blk, P.key0, P.key1);
+            r.w0 = b.x; r.w1 = b.y; r.w2 = b.z; r.w3 = b.w;
+            r.blk += 1u;
+        }
         r.left = 4u;
-        oc.add(2);
+        oc.add(2);                            // blocks counted when consumed
     }
     oc.add(1);
     const uint32_t x = r.w0;
@@ -208,10 +236,10 @@ __device__ __forceinline__ float jclamp(float x, float lo, float hi)
 }
 
 // ----------------------------------------------------------------- optics ----
-// Unpolarized Fresnel with TIR (src/optics.jl:10-19).  Returns true on TIR.
-__device__ __forceinline__ bool fresnel(float n1, float n2, float cosi, float &R)
+// Unpolarized Fresnel with TIR (src/optics.jl:10-19).  Returns true on TIR.  q = n1 / n2
+// is the same binary32 quotient for every call at an interface, divided once on the host.
+__device__ __forceinline__ bool fresnel(float q, float n1, float n2, float cosi, float &R)
 {
-    const float q = n1 / n2;
     const float sin2t = (q * q) * (1.0f - cosi * cosi);
     if (sin2t >= 1.0f) { R = 1.0f; return true; }
     const float cost = sqrtf(1.0f - sin2t);
@@ -315,7 +343,7 @@ __device__ __forceinline__ bool unified_surface(Rng &r, const KParams &P, OpCoun
         if (cosi <= 0.0f) continue;
         float R;
         oc.add(4);
-        const bool tir = fresnel(P.n_crystal, 1.0f, cosi, R);
+        const bool tir = fresnel(P.q_air, P.n_crystal, 1.0f, cosi, R);
         bool refl = tir;
         if (!tir) refl = randu<COUNT>(r, P, oc) < R;
         if (!refl) return true;
@@ -399,7 +427,7 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         // readout face: crystal -> coupling Fresnel, then PDE (src/kernel.jl:220-251)
         float R;
         oc.add(4);
-        const bool tir = fresnel(P.n_crystal, P.n_coupling, cosi, R);
+        const bool tir = fresnel(P.q_coup, P.n_crystal, P.n_coupling, cosi, R);
         bool refl = tir;
         if (!tir) refl = randu<COUNT>(r, P, oc) < R;
         if (refl) {
@@ -441,7 +469,7 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         } else {
             float R;
             oc.add(4);
-            const bool tir = fresnel(P.n_crystal, 1.0f, cosi, R);
+            const bool tir = fresnel(P.q_air, P.n_crystal, 1.0f, cosi, R);
             bool refl = tir;
             if (!tir) refl = randu<COUNT>(r, P, oc) < R;
             transmitted = !refl;

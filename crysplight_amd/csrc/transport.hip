@@ -101,23 +101,27 @@ transport_kernel(const KParams P, const KBuffers B)
     rng_init(rng, 0);
     int64_t k = 0;                                  // photon number of this lane
     int32_t ev = 0;                                 // its event (event mode)
+    int64_t my_dep = 0;                             // deposit of this lane's last photon
     int nscat = 0;
     bool alive = false;
 
     uint64_t pool_next = 0, pool_end = 0;           // wave-uniform work pool
+    int64_t pool_dep = 0;                           // deposit holding the pool chunk's start
     bool exhausted = false;
     // event mode reads the photon total the offsets scan produced
     const uint64_t N = (MODE == M_EVENTS && P.n_photons < 0) ? (uint64_t)B.dep_off[P.ndep]
                                                              : (uint64_t)P.n_photons;
+    const uint32_t wave_lanes = (uint32_t)__popcll(__ballot(true));
 
     for (;;) {
-        // ---- refill lanes whose photon ended --------------------------------------
+        // ---- refill idle lanes, in batches of at least refill_min ------------------
         const uint64_t need = __ballot(!alive);
-        if (need != 0ull && !exhausted) {
-            const uint32_t nneed = (uint32_t)__popcll(need);
+        const uint32_t nneed = (uint32_t)__popcll(need);
+        if (!exhausted && nneed != 0u && (nneed >= (uint32_t)P.refill_min || nneed == wave_lanes)) {
             const uint32_t rank = lanes_below(need);
             const uint64_t avail = pool_end - pool_next;
             uint64_t base2 = 0, end2 = 0;
+            int64_t new_dep = 0;
             if ((uint64_t)nneed > avail) {
                 uint64_t b = 0;
                 if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id()))
@@ -126,13 +130,15 @@ transport_kernel(const KParams P, const KBuffers B)
                 base2 = b;
                 end2 = b + (uint64_t)P.chunk < N ? b + (uint64_t)P.chunk : N;
                 if (b >= N) { base2 = end2 = N; exhausted = true; }
+                else if (MODE == M_EVENTS) new_dep = find_deposit(B.dep_off, P.ndep, (int64_t)b);
             }
             bool take = false;
+            int64_t start_dep = pool_dep;
             if (!alive) {
                 if ((uint64_t)rank < avail) { k = (int64_t)(pool_next + rank); take = true; }
                 else {
                     const uint64_t r2 = base2 + ((uint64_t)rank - avail);
-                    if (r2 < end2) { k = (int64_t)r2; take = true; }
+                    if (r2 < end2) { k = (int64_t)r2; take = true; start_dep = new_dep; }
                 }
             }
             if ((uint64_t)nneed <= avail) pool_next += nneed;
@@ -140,11 +146,15 @@ transport_kernel(const KParams P, const KBuffers B)
                 const uint64_t nx2 = base2 + ((uint64_t)nneed - avail);
                 pool_next = nx2 < end2 ? nx2 : end2;
                 pool_end = end2;
+                pool_dep = new_dep;
             }
             if (take) {
                 float x0, y0, z0, t0;
                 if (MODE == M_EVENTS) {
-                    const int64_t d = find_deposit(B.dep_off, P.ndep, k);
+                    // photons of a lane only move forward, and the chunk start bounds below
+                    int64_t d = start_dep > my_dep ? start_dep : my_dep;
+                    while (B.dep_off[d + 1] <= k) ++d;
+                    my_dep = d;
                     const float4 q = B.dep[d];
                     x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
                     ev = B.dep_event[d];
@@ -165,6 +175,7 @@ transport_kernel(const KParams P, const KBuffers B)
 
         // ---- one bounce step ------------------------------------------------------
         if (alive) {
+            rng_prefetch(rng, P.key0, P.key1);      // the step's Philox block, one site
             int lin = 0;
             const int st = step<COUNT>(p, rng, P, oc, lin, nscat);
             if (st != ST_FLYING) {
@@ -352,6 +363,7 @@ struct crysp_ctx {
     int wg = 256;
     int ppt = 4;
     int chunk = 256;
+    int refill_min = 16;
     bool timing = false;
     double kernel_ms = 0.0;
     int64_t n_launches = 0;
@@ -444,6 +456,8 @@ static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_par
     P.abs_len = op->abs_len_mm; P.ray_len = op->rayleigh_mm;
     P.wrap_R = op->wrap_R; P.sigma_alpha = op->sigma_alpha; P.pde = op->pde;
     P.inv_v = op->n_crystal / C_MM_NS;
+    P.q_coup = op->n_crystal / op->n_coupling;
+    P.q_air = op->n_crystal / 1.0f;
     P.cx = ro->cx; P.cy = ro->cy; P.r2 = ro->r2; P.sur_R = ro->sur_R;
     P.pitch = ro->grid.pitch_mm; P.nx = ro->grid.nx; P.ny = ro->grid.ny;
     P.tau_d = em ? em->tau_decay : 0.0f;
@@ -479,6 +493,7 @@ static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_byte
     const int wg = c->wg;
     const int grid = grid_for(c, P.n_photons, wg, c->ppt, lds_bytes);
     P.chunk = c->chunk;
+    P.refill_min = c->refill_min;
     void *cur;
     int rc = ensure(c, B_CURSOR, sizeof(unsigned long long), &cur);
     if (rc) return rc;
@@ -582,6 +597,16 @@ int crysp_set_schedule(crysp_ctx *c, int32_t wg, int32_t ppt)
     if (ppt < 1 || ppt > 64) return fail(CRYSP_EINVAL, "photons_per_thread must be 1..64");
     c->wg = wg;
     c->ppt = ppt;
+    return CRYSP_OK;
+}
+
+int crysp_set_queue(crysp_ctx *c, int32_t chunk, int32_t refill_min)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (chunk < 1 || chunk > (1 << 20)) return fail(CRYSP_EINVAL, "chunk must be 1..2^20");
+    if (refill_min < 1 || refill_min > 64) return fail(CRYSP_EINVAL, "refill_min must be 1..64");
+    c->chunk = chunk;
+    c->refill_min = refill_min;
     return CRYSP_OK;
 }
 
@@ -793,6 +818,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         B.opcounts = (unsigned long long *)ops;
     }
     P.chunk = c->chunk;
+    P.refill_min = c->refill_min;
     // the grid only sizes the persistent launch; the kernel drains whatever total it reads
     const int64_t n_hint = ntot_host >= 0 ? ntot_host : (c->last_n_hint > 0 ? c->last_n_hint : (int64_t)1 << 40);
     P.n_photons = -1;                               // read dep_off[ndep] on the device

@@ -154,6 +154,11 @@ int crysp_run_events(crysp_ctx *ctx, const crysp_box *box,
 int crysp_set_schedule(crysp_ctx *ctx, int32_t wg_size, int32_t photons_per_thread);
 int crysp_get_schedule(crysp_ctx *ctx, int32_t *wg_size, int32_t *photons_per_thread);
 
+/* Work-queue knobs (no reference counterpart): a wave takes `chunk` consecutive photons
+ * per atomic grab of the queue cursor, and hands new photons to its idle lanes once at
+ * least refill_min of them are idle (or all are).  Defaults 256 and 16. */
+int crysp_set_queue(crysp_ctx *ctx, int32_t chunk, int32_t refill_min);
+
 /* Autotuner (no reference counterpart): times crysp_run_events on the given deposits
  * for every (wg_sizes[a], ppts[b]) with `reps` repetitions each, keeps the fastest as
  * the context's schedule and writes the best-of-reps kernel time [ms] of every pair
