@@ -102,8 +102,17 @@ struct OpCount {
     __device__ __forceinline__ void add(int k, uint32_t d = 1u) { if (COUNT) v[k] += d; }
 };
 
-__device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1)
+// Top of a step: an empty current block takes the look-ahead one (counted as consumed),
+// then a missing look-ahead block is generated, so a step starts with 5..8 words.
+template <bool COUNT>
+__device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1, OpCount<COUNT> &oc)
 {
+    if (r.left == 0u && r.has_next) {
+        r.w0 = r.n0; r.w1 = r.n1; r.w2 = r.n2; r.w3 = r.n3;
+        r.has_next = 0u;
+        r.left = 4u;
+        oc.add(2);
+    }
     if (!r.has_next) {
         const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, k0, k1);
         r.n0 = b.x; r.n1 = b.y; r.n2 = b.z; r.n3 = b.w;
@@ -251,10 +260,15 @@ __device__ __forceinline__ bool fresnel(float q, float n1, float n2, float cosi,
 
 __device__ __forceinline__ int face_axis(int f) { return f <= 2 ? 1 : (f <= 4 ? 2 : 3); }
 
+// Specular flip of the face-normal component (src/optics.jl:25-28), written as three
+// selects so the direction stays in registers (an axis-indexed write would put it in
+// scratch memory).
 __device__ __forceinline__ void specular(float &ux, float &uy, float &uz, int face)
 {
     const int a = face_axis(face);
-    if (a == 1) ux = -ux; else if (a == 2) uy = -uy; else uz = -uz;
+    ux = a == 1 ? -ux : ux;
+    uy = a == 2 ? -uy : uy;
+    uz = a == 3 ? -uz : uz;
 }
 
 // Lambertian re-injection about the inward normal (src/optics.jl:118-138).

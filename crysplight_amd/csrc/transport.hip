@@ -41,6 +41,7 @@ struct KBuffers {
     uint8_t *status; int16_t *idx; float *tdet; int32_t *bnc;   // records
     uint32_t *counts; uint32_t *tkeys; unsigned long long *counters;  // accumulate
     int32_t *maps; uint32_t *ev_tkeys;              // events
+    unsigned long long *ev_tsum;                    // events, optional: sum of (t - t0) in 2^-10 ns
     unsigned long long *cursor;                     // work-queue cursor (zeroed)
     unsigned long long *opcounts;                   // 10 op counters (COUNT only)
     unsigned int *flag;                             // events: input error flags
@@ -82,10 +83,12 @@ __global__ void __launch_bounds__(1024)
 transport_kernel(const KParams P, const KBuffers B)
 {
     extern __shared__ uint32_t lds[];               // accumulate: hist + time keys
+    __shared__ unsigned long long lds_ctr[9];       // accumulate: the Accumulator counters
     const int nxy = P.nx * P.ny;
     if (MODE == M_ACCUM) {
         for (int w = threadIdx.x; w < P.hist_words + nxy; w += blockDim.x)
             lds[w] = w < P.hist_words ? 0u : KEY_INF;
+        if (threadIdx.x < 9) lds_ctr[threadIdx.x] = 0ull;
         __syncthreads();
     }
 
@@ -93,14 +96,12 @@ transport_kernel(const KParams P, const KBuffers B)
 #pragma unroll
     for (int k = 0; k < 10; ++k) oc.v[k] = 0u;
 
-    unsigned long long c_det = 0, c_front = 0, c_bulk = 0, c_wall = 0, c_sipm = 0, c_cap = 0,
-                       c_sur = 0, c_scat = 0, c_bsum = 0;
-
     Photon p{};
     Rng rng;
     rng_init(rng, 0);
     int64_t k = 0;                                  // photon number of this lane
     int32_t ev = 0;                                 // its event (event mode)
+    float t_start = 0.0f;                           // its deposit time (event mode)
     int64_t my_dep = 0;                             // deposit of this lane's last photon
     int nscat = 0;
     bool alive = false;
@@ -158,6 +159,7 @@ transport_kernel(const KParams P, const KBuffers B)
                     const float4 q = B.dep[d];
                     x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
                     ev = B.dep_event[d];
+                    t_start = t0;
                 } else {
                     const int64_t j = P.npos == 1 ? 0 : k % (int64_t)P.npos;
                     x0 = B.px[j]; y0 = B.py[j]; z0 = B.pz[j];
@@ -175,7 +177,7 @@ transport_kernel(const KParams P, const KBuffers B)
 
         // ---- one bounce step ------------------------------------------------------
         if (alive) {
-            rng_prefetch(rng, P.key0, P.key1);      // the step's Philox block, one site
+            rng_prefetch<COUNT>(rng, P.key0, P.key1, oc);  // the step's Philox block, one site
             int lin = 0;
             const int st = step<COUNT>(p, rng, P, oc, lin, nscat);
             if (st != ST_FLYING) {
@@ -186,29 +188,33 @@ transport_kernel(const KParams P, const KBuffers B)
                     if (B.tdet) B.tdet[k] = p.t;
                     if (B.bnc) B.bnc[k] = p.bounces;
                 } else if (MODE == M_ACCUM) {
-                    c_scat += (unsigned)nscat;
+                    // counter slots: ndet 0, ndet_front 1, bulk 2, wall 3, sipm 4, cap 5,
+                    // surround 6, nscat 7, sum_bounces_det 8 (the statuses map onto 0, 2..6)
+                    if (nscat) atomicAdd(&lds_ctr[7], (unsigned long long)nscat);
                     if (st == ST_DETECTED) {
                         int l = lin;
-                        if (l < 0) { ++c_front; l = -l; }
+                        if (l < 0) { atomicAdd(&lds_ctr[1], 1ull); l = -l; }
                         const int cell = l - 1;     // (ii-1) + nx*(jj-1)
                         const float q = p.t / P.bin_ns;
                         int bin = (q >= (float)P.nbins) ? P.nbins : (int)q;  // 0-based
                         bin = bin < 0 ? 0 : bin;
                         atomicAdd(&lds[cell + nxy * bin], 1u);
                         atomicMin(&lds[P.hist_words + cell], time_key(p.t));
-                        ++c_det;
-                        c_bsum += (unsigned)p.bounces;
-                    } else if (st == ST_ABS_BULK) ++c_bulk;
-                    else if (st == ST_ABS_WALL) ++c_wall;
-                    else if (st == ST_ABS_SIPM) ++c_sipm;
-                    else if (st == ST_ABS_SUR) ++c_sur;
-                    else ++c_cap;
+                        atomicAdd(&lds_ctr[8], (unsigned long long)p.bounces);
+                    }
+                    const int slot = st == ST_DETECTED ? 0 : (st == ST_CAP ? 5 : (st == ST_ABS_SUR ? 6 : st));
+                    atomicAdd(&lds_ctr[slot], 1ull);
                 } else {
                     if (st == ST_DETECTED && (uint32_t)ev < (uint32_t)P.nev) {
                         const int l = lin < 0 ? -lin : lin;
                         const int64_t cell = (int64_t)(l - 1) + (int64_t)nxy * ev;
                         atomicAdd(&B.maps[cell], 1);
                         atomicMin(&B.ev_tkeys[cell], time_key(p.t));
+                        if (B.ev_tsum) {
+                            const float dt = p.t - t_start;
+                            atomicAdd(&B.ev_tsum[cell],
+                                      (unsigned long long)(dt > 0.0f ? dt * 1024.0f : 0.0f));
+                        }
                     }
                 }
             }
@@ -223,14 +229,9 @@ transport_kernel(const KParams P, const KBuffers B)
         }
     }
     if (MODE == M_ACCUM) {
-        unsigned long long vals[9] = { c_det, c_front, c_bulk, c_wall, c_sipm, c_cap, c_sur,
-                                       c_scat, c_bsum };
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const unsigned long long s = wave_sum(vals[q]);
-            if (__lane_id() == 0 && s) atomicAdd(&B.counters[q], s);
-        }
         __syncthreads();
+        if (threadIdx.x < 9 && lds_ctr[threadIdx.x])
+            atomicAdd(&B.counters[threadIdx.x], lds_ctr[threadIdx.x]);
         for (int w = threadIdx.x; w < P.hist_words; w += blockDim.x)
             if (lds[w]) atomicAdd(&B.counts[w], lds[w]);
         for (int w = threadIdx.x; w < nxy; w += blockDim.x)
@@ -757,7 +758,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                        const crysp_readout *ro, const float *dep_xyzt, const int32_t *dep_nph,
                        const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
                        uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
-                       float *tmin, int32_t *npe, int64_t *n_out, bool use_graph)
+                       float *tmin, int32_t *npe, int64_t *n_out, bool use_graph,
+                       unsigned long long *tsum_dev = nullptr)
 {
     if (!c) return fail(CRYSP_EINVAL, "null context");
     if (ndep < 0 || nev < 0) return fail(CRYSP_EINVAL, "negative sizes");
@@ -809,6 +811,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         return rc;
     B.maps = (int32_t *)dmaps;
     B.ev_tkeys = (uint32_t *)dkeys;
+    B.ev_tsum = tsum_dev;
     void *cur;
     if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
     B.cursor = (unsigned long long *)cur;
@@ -846,6 +849,9 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                            (int64_t)cells);
         hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
                            KEY_INF, (int64_t)cells);
+        if (tsum_dev)
+            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)tsum_dev, 0u,
+                               (int64_t)cells * 2);
         if (c->count_ops)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
         if (ndep > 0) {
@@ -920,6 +926,37 @@ int crysp_run_events(crysp_ctx *c, const crysp_box *box, const crysp_optical_par
 {
     return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
                        maxb, em, maps, tmin, npe, n_out, true);
+}
+
+int crysp_light_maps(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                     const crysp_readout *ro, const float *xyz, const int32_t *nph, int64_t npts,
+                     uint64_t seed, uint64_t pid0, int32_t maxb, const crysp_emission *em,
+                     int32_t *counts, uint64_t *tsum_fx)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (npts < 0 || (npts > 0 && (!xyz || !nph || !counts))) return fail(CRYSP_EINVAL, "bad arguments");
+    if (!ro) return fail(CRYSP_EINVAL, "null readout");
+    if (npts == 0) return CRYSP_OK;
+    const int nxy = ro->grid.nx * ro->grid.ny;
+    std::vector<float> dep((size_t)npts * 4);
+    std::vector<int32_t> ev((size_t)npts);
+    for (int64_t i = 0; i < npts; ++i) {
+        dep[4 * i] = xyz[3 * i]; dep[4 * i + 1] = xyz[3 * i + 1]; dep[4 * i + 2] = xyz[3 * i + 2];
+        dep[4 * i + 3] = 0.0f;
+        ev[i] = (int32_t)i;
+    }
+    std::vector<float> tmin((size_t)npts * nxy);
+    std::vector<int32_t> npe((size_t)npts);
+    void *dts = nullptr;
+    int rc;
+    if (tsum_fx && (rc = ensure(c, B_MISC2, (size_t)npts * nxy * 8, &dts))) return rc;
+    rc = events_impl(c, box, op, ro, dep.data(), nph, ev.data(), npts, npts, seed, pid0, maxb, em,
+                     counts, tmin.data(), npe.data(), nullptr, false, (unsigned long long *)dts);
+    if (rc) return rc;
+    if (tsum_fx) {
+        HIPCHK(hipMemcpy(tsum_fx, dts, (size_t)npts * nxy * 8, hipMemcpyDeviceToHost));
+    }
+    return CRYSP_OK;
 }
 
 int crysp_autotune_events(crysp_ctx *c, const crysp_box *box,

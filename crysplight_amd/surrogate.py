@@ -1,0 +1,380 @@
+"""Learned light-response surrogate: an MLP from (deposit position, energy) to the mean
+SiPM count and arrival-time maps, trained on Monte Carlo targets of this library's own
+transport, and Poisson sampling of event maps from it.  No reference counterpart.
+
+Model.  For a deposit of energy E at x, the photon count is Poisson(Y E / 1000) and each
+photon lands in SiPM c with probability p_c(x), so the SiPM counts are independent
+Poisson(Y E / 1000 * p_c(x)) (Poisson thinning): the mean count map determines the
+count distribution exactly.  The network (4 -> 256 -> 256 -> 256 -> 128, GELU) predicts
+the per-photon map as log-rates log p_c (outputs 0..63, trained with a Poisson negative
+log-likelihood of the observed counts, which is unbiased and weighs relative errors
+evenly) and the mean arrival time of a detected photon after the deposit (outputs
+64..127, standardized, squared error weighted by the counts); the mean count map is
+Y E / 1000 * p.  Energy enters as the fourth input.
+
+Inference.  `predict` runs the network (BF16 MFMA kernels).  For event sampling the
+network is evaluated once on a regular grid over the crystal (`bake`) and deposits
+interpolate it trilinearly on the device, so the per-event cost no longer includes an
+MLP evaluation (0.33 MFLOP per deposit).  Deposits within `fallback_mm` of a face go to
+the Monte Carlo instead (`split_deposits`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .events import Deposits, transport_events
+from .params import Box, Emission, OpticalParams, SipmGrid, readout
+
+
+def _bind():
+    lib = _lib.load()
+    if getattr(lib, "_mlp_bound", False):
+        return lib
+    vp = C.c_void_p
+    lib.crysp_mlp_create.restype = vp
+    lib.crysp_mlp_create.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+    lib.crysp_mlp_destroy.argtypes = [vp]
+    lib.crysp_mlp_last_error.restype = C.c_char_p
+    lib.crysp_mlp_num_params.restype = C.c_int64
+    lib.crysp_mlp_num_params.argtypes = [vp]
+    lib.crysp_mlp_flops_per_row.restype = C.c_double
+    lib.crysp_mlp_flops_per_row.argtypes = [vp]
+    lib.crysp_mlp_get_params.argtypes = [vp, vp]
+    lib.crysp_mlp_set_params.argtypes = [vp, vp]
+    lib.crysp_mlp_set_input_norm.argtypes = [vp, vp, vp]
+    lib.crysp_mlp_set_poisson_outputs.argtypes = [vp, C.c_int32]
+    lib.crysp_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
+    lib.crysp_mlp_gradient.argtypes = [vp, vp, vp, vp, C.c_int32, vp, vp]
+    lib.crysp_mlp_train_batches.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int32, C.c_float,
+                                            C.c_float, C.c_float, C.c_float, C.c_float,
+                                            C.c_float, vp]
+    lib.crysp_surrogate_sample_events.argtypes = [C.c_int, vp, C.c_int32, C.c_int32, C.c_int32,
+                                                  C.c_int32, C.POINTER(Box), vp, vp, vp, vp,
+                                                  C.c_int64, C.c_int64, C.c_double, C.c_uint64,
+                                                  vp, vp, vp]
+    lib.crysp_debug_gemm.argtypes = [C.c_int, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp,
+                                     C.c_int32, vp]
+    lib.crysp_light_maps.argtypes = [vp, C.POINTER(Box), C.POINTER(OpticalParams), vp, vp, vp,
+                                     C.c_int64, C.c_uint64, C.c_uint64, C.c_int32,
+                                     C.POINTER(Emission), vp, vp]
+    lib._mlp_bound = True
+    return lib
+
+
+def _mcheck(rc, what):
+    if rc != 0:
+        raise _lib.CryspError(f"{what} failed ({rc}): " + _bind().crysp_mlp_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(_lib.ptr(a))
+
+
+class MLP:
+    """The network on the device: BF16 MFMA forward/backward, FP32 master weights, Adam."""
+
+    def __init__(self, n_in=4, hidden=256, n_hidden_layers=3, n_out=128, seed=1, device=0):
+        lib = _bind()
+        h = lib.crysp_mlp_create(device, n_in, hidden, n_hidden_layers, n_out, seed)
+        if not h:
+            raise _lib.CryspError("crysp_mlp_create: " + lib.crysp_mlp_last_error().decode())
+        self._h = h
+        self.n_in, self.hidden, self.n_hidden_layers, self.n_out = n_in, hidden, n_hidden_layers, n_out
+        self.shapes = [(b, a) for a, b in zip([n_in] + [hidden] * n_hidden_layers,
+                                              [hidden] * n_hidden_layers + [n_out])]
+        self.shift = np.zeros(n_in, np.float32)
+        self.scale = np.ones(n_in, np.float32)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _bind().crysp_mlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def n_params(self) -> int:
+        return int(_bind().crysp_mlp_num_params(self._h))
+
+    @property
+    def flops_per_row(self) -> float:
+        return float(_bind().crysp_mlp_flops_per_row(self._h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _mcheck(_bind().crysp_mlp_get_params(self._h, p.ctypes.data), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _mcheck(_bind().crysp_mlp_set_params(self._h, p.ctypes.data), "set_params")
+
+    def set_input_norm(self, shift, scale):
+        self.shift = np.ascontiguousarray(shift, np.float32)
+        self.scale = np.ascontiguousarray(scale, np.float32)
+        _mcheck(_bind().crysp_mlp_set_input_norm(self._h, self.shift.ctypes.data,
+                                                 self.scale.ctypes.data), "set_input_norm")
+
+    def set_poisson_outputs(self, n_pois: int):
+        _mcheck(_bind().crysp_mlp_set_poisson_outputs(self._h, int(n_pois)), "set_poisson_outputs")
+
+    def forward(self, X, out=None):
+        n = int(X.shape[0])
+        if out is None:
+            Xh = np.ascontiguousarray(X, np.float32)
+            out = np.zeros((n, self.n_out), np.float32)
+            _mcheck(_bind().crysp_mlp_forward(self._h, Xh.ctypes.data, n, out.ctypes.data), "forward")
+            return out
+        _mcheck(_bind().crysp_mlp_forward(self._h, _ptr(X), n, _ptr(out)), "forward")
+        return out
+
+    def gradient(self, X, T, W=None):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        W = None if W is None else np.ascontiguousarray(W, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        _mcheck(_bind().crysp_mlp_gradient(self._h, X.ctypes.data, T.ctypes.data,
+                                           None if W is None else W.ctypes.data, X.shape[0],
+                                           g.ctypes.data, loss.ctypes.data), "gradient")
+        return float(loss[0]), g
+
+    def train_batches(self, X, T, W, n_batches, batch, lr=1e-3, betas=(0.9, 0.999), eps=1e-8,
+                      weight_decay=0.0, mean_row_weight=1.0):
+        """Adam steps over consecutive rows of device tensors X, T, W."""
+        losses = np.zeros(max(1, n_batches), np.float32)
+        _mcheck(_bind().crysp_mlp_train_batches(self._h, _ptr(X), _ptr(T), _ptr(W), int(n_batches),
+                                                int(batch), lr, betas[0], betas[1], eps,
+                                                weight_decay, mean_row_weight,
+                                                losses.ctypes.data), "train_batches")
+        return losses[:n_batches]
+
+
+def debug_gemm(A, B, a_t=False, b_t=False, device=0):
+    """C = op(A) op(B) through the device GEMM; A, B given as float arrays (rounded to
+    bf16).  Shapes: op(A) [M x K], op(B) [K x N]."""
+    import torch
+    a = torch.from_numpy(np.ascontiguousarray(A, np.float32)).to(torch.bfloat16).view(torch.int16).numpy()
+    b = torch.from_numpy(np.ascontiguousarray(B, np.float32)).to(torch.bfloat16).view(torch.int16).numpy()
+    M = A.shape[1] if a_t else A.shape[0]
+    K = A.shape[0] if a_t else A.shape[1]
+    N = B.shape[0] if b_t else B.shape[1]
+    Cm = np.zeros((M, N), np.float32)
+    _mcheck(_bind().crysp_debug_gemm(device, M, N, K, a.ctypes.data, int(a_t), b.ctypes.data,
+                                     int(b_t), Cm.ctypes.data), "debug_gemm")
+    return Cm
+
+
+def mc_light_maps(box: Box, op: OpticalParams, ro, xyz, nph, *, seed: int, tau_ns: float,
+                  tau_rise_ns: float = 0.0, pid0: int = 0, max_bounces: int = 100_000, ctx=None):
+    """Monte Carlo light-response maps of point sources: counts [n, nx*ny] (int32) and
+    the exact sums of arrival times [n, nx*ny] in ns (binary64 from integer sums)."""
+    ctx = ctx or _lib.default_context()
+    ro = readout(ro) if isinstance(ro, SipmGrid) else ro
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    n = xyz.shape[0]
+    nph = np.ascontiguousarray(np.broadcast_to(np.asarray(nph, np.int32), (n,)))
+    nxy = ro.grid.nx * ro.grid.ny
+    counts = np.zeros((n, nxy), np.int32)
+    tsum = np.zeros((n, nxy), np.uint64)
+    em = Emission(tau_ns, tau_rise_ns)
+    rc = _bind().crysp_light_maps(ctx.handle, C.byref(box), C.byref(op), C.c_void_p(C.addressof(ro)),
+                                  xyz.ctypes.data, nph.ctypes.data, n, int(seed), int(pid0),
+                                  int(max_bounces), C.byref(em), counts.ctypes.data,
+                                  tsum.ctypes.data)
+    _lib.check(rc, "crysp_light_maps")
+    return counts, tsum.astype(np.float64) / 1024.0
+
+
+@dataclass
+class TrainingSet:
+    X: np.ndarray        # (n, 4) x, y, z [mm], E [keV]
+    counts: np.ndarray   # (n, 64) int32
+    tsum: np.ndarray     # (n, 64) float64 ns
+    nph: int
+
+    @property
+    def p(self):
+        return self.counts / float(self.nph)
+
+    @property
+    def tmean(self):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return np.where(self.counts > 0, self.tsum / np.maximum(self.counts, 1), np.nan)
+
+
+class LightSurrogate:
+    """Surrogate of one crystal/optics configuration."""
+
+    def __init__(self, box: Box, op: OpticalParams, ro, *, tau_ns: float,
+                 tau_rise_ns: float = 0.0, e_range=(10.0, 511.0), hidden=256, n_hidden_layers=3,
+                 seed=1, device=0, log_depth_mm: float = 0.0):
+        self.box, self.op = box, op
+        self.ro = readout(ro) if isinstance(ro, SipmGrid) else ro
+        self.ncell = self.ro.grid.nx * self.ro.grid.ny
+        self.tau_ns, self.tau_rise_ns = float(tau_ns), float(tau_rise_ns)
+        self.e_range = tuple(float(v) for v in e_range)
+        self.net = MLP(4, hidden, n_hidden_layers, 2 * self.ncell, seed=seed, device=device)
+        # log_depth_mm > 0: the depth input is -log(Lz - z + log_depth_mm), which stretches
+        # the region next to the readout face where the maps sharpen like 1 / distance^2
+        self.log_depth_mm = float(log_depth_mm)
+        L = np.asarray(box.L, np.float32)
+        lo = np.array([0, 0, 0, self.e_range[0]], np.float32)
+        hi = np.array([L[0], L[1], L[2], self.e_range[1]], np.float32)
+        if self.log_depth_mm > 0:
+            lo[2], hi[2] = self._depth(np.float32(0.0)), self._depth(L[2])
+        self.net.set_input_norm((lo + hi) / 2, 2.0 / (hi - lo))
+        self.net.set_poisson_outputs(self.ncell)
+        self.pois_w = 1.0
+        self.t_shift = self.tau_ns + self.tau_rise_ns
+        self.t_scale = max(self.tau_ns + self.tau_rise_ns, 1.0)
+        self.device = device
+
+    def _depth(self, z):
+        return -np.log(np.float32(self.box.L[2]) - z + np.float32(self.log_depth_mm))
+
+    def features(self, xyz, e_kev):
+        """Network inputs (x, y, depth, E) for positions [n, 3] and energies [n]."""
+        X = np.column_stack([np.asarray(xyz, np.float32).reshape(-1, 3),
+                             np.asarray(e_kev, np.float32).reshape(-1)]).astype(np.float32)
+        if self.log_depth_mm > 0:
+            X[:, 2] = self._depth(X[:, 2])
+        return X
+
+    # ------------------------------------------------------------------ data ----
+    def generate(self, n_samples: int, photons_per_sample: int, *, seed: int, pid0: int = 0,
+                 ctx=None) -> TrainingSet:
+        """Positions uniform in the crystal and energies uniform in e_range (Philox
+        domain 2), each traced at photons_per_sample photons."""
+        from .events import synthetic_deposits
+        d = synthetic_deposits(self.box, n_events=n_samples, seed=seed, e_kev=self.e_range,
+                               n_fixed=photons_per_sample)
+        X = np.column_stack([d.xyzt[:, :3], d.e_kev]).astype(np.float32)
+        counts, tsum = mc_light_maps(self.box, self.op, self.ro, X[:, :3], photons_per_sample,
+                                     seed=seed, tau_ns=self.tau_ns, tau_rise_ns=self.tau_rise_ns,
+                                     pid0=pid0, ctx=ctx)
+        return TrainingSet(X, counts, tsum, int(photons_per_sample))
+
+    def targets(self, ts: TrainingSet):
+        """Observed per-photon rates (Poisson outputs, weight 1 / mean rate so gradients
+        read as relative errors) and standardized mean times (weight c / (c + 20))."""
+        p = ts.p
+        tm = ts.tmean
+        T = np.empty((ts.X.shape[0], 2 * self.ncell), np.float32)
+        T[:, :self.ncell] = p
+        T[:, self.ncell:] = np.nan_to_num((tm - self.t_shift) / self.t_scale, nan=0.0)
+        W = np.empty_like(T)
+        self.pois_w = float(1.0 / max(p.mean(), 1e-12))
+        W[:, :self.ncell] = self.pois_w
+        c = ts.counts.astype(np.float32)
+        W[:, self.ncell:] = 0.25 * c / (c + 20.0)
+        return T, W
+
+    def init_output_bias(self, ts: TrainingSet):
+        """Start the log-rate outputs at the log of each cell's mean rate."""
+        par = self.net.get_params()
+        n_out = 2 * self.ncell
+        b = par.size - n_out
+        par[b:b + self.ncell] = np.log(np.maximum(ts.p.mean(0), 1e-9))
+        par[b + self.ncell:] = 0.0
+        self.net.set_params(par)
+
+    # ------------------------------------------------------------------ train ---
+    def fit(self, ts: TrainingSet, *, epochs: int = 10, batch: int = 4096, lr: float = 2e-3,
+            lr_final: float = 1e-4, seed: int = 0, verbose: bool = False):
+        """Adam over shuffled mini-batches, learning rate decayed geometrically per epoch.
+        Returns the per-step losses."""
+        import torch
+        T, W = self.targets(ts)
+        if self.net.get_params()[-2 * self.ncell:].any() == 0:
+            self.init_output_bias(ts)
+        n = (ts.X.shape[0] // batch) * batch
+        if n == 0:
+            raise ValueError("training set smaller than one batch")
+        dev = torch.device("cuda", self.device)
+        X_d = torch.from_numpy(self.features(ts.X[:, :3], ts.X[:, 3])).to(dev)
+        T_d = torch.from_numpy(T).to(dev)
+        W_d = torch.from_numpy(W).to(dev)
+        mrw = float(W.sum(1).mean() / W.shape[1])
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        losses = []
+        for ep in range(epochs):
+            lr_ep = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
+            perm = torch.randperm(ts.X.shape[0], generator=g)[:n].to(dev)
+            Xb = X_d.index_select(0, perm).contiguous()
+            Tb = T_d.index_select(0, perm).contiguous()
+            Wb = W_d.index_select(0, perm).contiguous()
+            torch.cuda.synchronize()
+            l = self.net.train_batches(Xb, Tb, Wb, n // batch, batch, lr=lr_ep,
+                                       mean_row_weight=mrw)
+            losses.append(l)
+            if verbose:
+                print(f"epoch {ep}: lr {lr_ep:.2e} loss {l.mean():.5f}", flush=True)
+        return np.concatenate(losses)
+
+    # -------------------------------------------------------------- inference ----
+    def predict(self, xyz, e_kev):
+        """Per-photon detection maps p [n, 64] and mean arrival times [n, 64] (ns after
+        the deposit)."""
+        Y = self.net.forward(self.features(xyz, e_kev))
+        p = np.exp(Y[:, :self.ncell])
+        t = Y[:, self.ncell:] * self.t_scale + self.t_shift
+        return p, t
+
+    def bake(self, shape=(48, 48, 40), e_kev=None):
+        """The network on a regular (gx, gy, gz) grid spanning the crystal, at one energy
+        (default mid-range): float32 [gz, gy, gx, 2, ncell] (p, then mean time)."""
+        gx, gy, gz = shape
+        L = np.asarray(self.box.L, np.float64)
+        xs = np.linspace(0, L[0], gx)
+        ys = np.linspace(0, L[1], gy)
+        zs = np.linspace(0, L[2], gz)
+        Z, Yg, Xg = np.meshgrid(zs, ys, xs, indexing="ij")
+        e = np.full(Xg.size, np.mean(self.e_range) if e_kev is None else e_kev, np.float32)
+        p, t = self.predict(np.column_stack([Xg.ravel(), Yg.ravel(), Z.ravel()]), e)
+        grid = np.empty((gz, gy, gx, 2, self.ncell), np.float32)
+        grid[..., 0, :] = p.reshape(gz, gy, gx, self.ncell)
+        grid[..., 1, :] = t.reshape(gz, gy, gx, self.ncell)
+        self.grid = grid
+        return grid
+
+
+def split_deposits(dep: Deposits, box: Box, fallback_mm: float = 1.0) -> np.ndarray:
+    """True where a deposit is at least fallback_mm from every face (surrogate region)."""
+    L = np.asarray(box.L, np.float32)
+    x = dep.xyzt[:, :3]
+    return np.all((x >= fallback_mm) & (x <= L - fallback_mm), axis=1)
+
+
+def sample_events(grid, box: Box, dep: Deposits, ev_first, use, yield_per_mev: float, *,
+                  seed: int, device: int = 0, out=None):
+    """Poisson event maps from a baked grid for the deposits with use != 0."""
+    gz, gy, gx, two, ncell = grid.shape
+    nev = dep.n_events
+    lib = _bind()
+    if out is None:
+        maps = np.zeros((nev, ncell), np.int32)
+        tmean = np.zeros((nev, ncell), np.float32)
+    else:
+        maps, tmean = out
+    use8 = np.ascontiguousarray(use, np.uint8) if isinstance(use, np.ndarray) else use
+    rc = lib.crysp_surrogate_sample_events(
+        device, _ptr(grid), gx, gy, gz, ncell, C.byref(box), _ptr(dep.xyzt), _ptr(dep.e_kev),
+        _ptr(use8), _ptr(ev_first), int(dep.xyzt.shape[0]), nev, float(yield_per_mev),
+        int(seed), _ptr(maps), _ptr(tmean), None)
+    _mcheck(rc, "crysp_surrogate_sample_events")
+    return maps, tmean
+
+
+def count_error(p_pred, p_true, floor_frac=0.0):
+    """Mean per-SiPM relative count error, count-weighted per point:
+    sum_c |p_pred - p_true| / sum_c p_true, averaged over points."""
+    num = np.abs(p_pred - p_true).sum(1)
+    den = p_true.sum(1)
+    return float(np.mean(num / np.maximum(den, 1e-30)))

@@ -147,6 +147,16 @@ int crysp_run_events(crysp_ctx *ctx, const crysp_box *box,
                      const crysp_emission *em,
                      int32_t *maps, float *tmin, int32_t *npe, int64_t *n_photons_out);
 
+/* Light-response maps (no reference counterpart; the surrogate's Monte Carlo targets):
+ * npts point sources at xyz[3i..3i+2] with nph[i] photons each (t0 = 0), one event per
+ * point through the event mode.  counts: i32 (nx, ny, npts); tsum_fx (optional): u64
+ * (nx, ny, npts) sums of the detected photons' arrival times in units of 2^-10 ns, so
+ * mean times are exact integer sums divided on the host in binary64. */
+int crysp_light_maps(crysp_ctx *ctx, const crysp_box *box, const crysp_optical_params *op,
+                     const crysp_readout *ro, const float *xyz, const int32_t *nph, int64_t npts,
+                     uint64_t seed, uint64_t pid0, int32_t max_bounces, const crysp_emission *em,
+                     int32_t *counts, uint64_t *tsum_fx);
+
 /* Launch schedule of the transport kernels: workgroup size (64..1024, multiple of 64)
  * and photons per thread (1..8; sets the persistent grid to
  * ceil(n_photons / (wg * ppt)) workgroups, capped at what the device keeps resident).
@@ -250,6 +260,56 @@ int crysp_synthetic_deposits(int64_t n_events, uint64_t seed, const float *lo3,
 /* rand_poisson (src/events.jl:11-22: Knuth for lambda <= 50, rounded Gaussian above)
  * on n independent Philox streams, for the sampler's tests. */
 int crysp_poisson_samples(uint64_t seed, double lambda, int64_t n, int64_t *out);
+
+/* ---- learned light-response surrogate (no reference counterpart) ---------------
+ * An MLP n_in -> hidden x n_hidden_layers -> n_out with GELU, trained and evaluated by
+ * BF16 MFMA GEMMs (v_mfma_f32_32x32x16_bf16) accumulating in FP32, FP32 master weights
+ * and Adam.  Parameters are a flat FP32 vector: per layer W [out][in] then b [out]. */
+typedef struct crysp_mlp crysp_mlp;
+
+crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_hidden_layers,
+                            int32_t n_out, uint64_t seed);
+void crysp_mlp_destroy(crysp_mlp *mlp);
+const char *crysp_mlp_last_error(void);
+int64_t crysp_mlp_num_params(const crysp_mlp *mlp);
+double crysp_mlp_flops_per_row(const crysp_mlp *mlp);
+int crysp_mlp_get_params(crysp_mlp *mlp, float *params);
+int crysp_mlp_set_params(crysp_mlp *mlp, const float *params);
+/* x' = (x - shift) * scale, applied to the n_in inputs inside the network */
+int crysp_mlp_set_input_norm(crysp_mlp *mlp, const float *shift, const float *scale);
+/* The first n_pois outputs are log-rates trained with a Poisson negative log-likelihood,
+ * w (e^y - t y) for an observed rate t; the rest with a weighted squared error. */
+int crysp_mlp_set_poisson_outputs(crysp_mlp *mlp, int32_t n_pois);
+/* Y [M x n_out] = MLP(X [M x n_in]); host or device pointers */
+int crysp_mlp_forward(crysp_mlp *mlp, const float *X, int64_t M, float *Y);
+/* Gradient of the loss (normalized by sum W) over one batch (W may be NULL: all ones),
+ * without an update: grads [num_params] and loss (host pointers, optional). */
+int crysp_mlp_gradient(crysp_mlp *mlp, const float *X, const float *T, const float *W,
+                       int32_t M, float *grads, float *loss);
+/* n_batches Adam steps on consecutive device-resident rows (batch a multiple of 64);
+ * mean_row_weight = mean over rows of sum(W) / n_out, the loss normalization.
+ * losses[b] (host, optional) receives each step's loss. */
+int crysp_mlp_train_batches(crysp_mlp *mlp, const float *X, const float *T, const float *W,
+                            int64_t n_batches, int32_t batch, float lr, float beta1, float beta2,
+                            float eps, float weight_decay, float mean_row_weight, float *losses);
+
+/* Events from a surrogate baked on a regular grid: grid [gz][gy][gx][2][ncell] holds per
+ * grid point the per-photon detection probability of every cell and the mean arrival
+ * time (ns after the deposit); for event e with deposits [ev_first[e], ev_first[e+1])
+ * and use[d] != 0, lambda_c = sum_d yield * E_d / 1000 * p_c(x_d) (trilinear) and
+ * maps[e][c] ~ Poisson(lambda_c) (src/events.jl:11-22 algorithm, Philox domain 4);
+ * tmean/lam (optional) get the count-weighted mean time and lambda. */
+int crysp_surrogate_sample_events(int device, const float *grid, int32_t gx, int32_t gy,
+                                  int32_t gz, int32_t ncell, const crysp_box *box,
+                                  const float *dep_xyzt, const float *dep_e_kev,
+                                  const uint8_t *use, const int64_t *ev_first, int64_t ndep,
+                                  int64_t nev, double yield_per_mev, uint64_t seed,
+                                  int32_t *maps, float *tmean, float *lam);
+
+/* Debug: C [M x N] = A * B in bf16 on the GEMM kernel (A [M][K], or [K][M] when a_t;
+ * B [K][N], or [N][K] when b_t), for the MFMA lane-layout check. */
+int crysp_debug_gemm(int device, int32_t M, int32_t N, int32_t K, const uint16_t *A_bf16,
+                     int32_t a_t, const uint16_t *B_bf16, int32_t b_t, float *C);
 
 #ifdef __cplusplus
 }

