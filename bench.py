@@ -48,6 +48,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate training / validation / event comparison")
+    ap.add_argument("--surrogate-events", type=int, default=1_000_000)
+    ap.add_argument("--surrogate-samples", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this size once (a plain copy "
                          "kernel of known traffic, to scale rocprofv3 FETCH/WRITE_SIZE)")
@@ -94,6 +98,10 @@ def main() -> int:
     ctx.set_op_counting(False)
     ops_per_launch = opmodel.algorithmic_ops(counts)
     n_photons = r.n_photons
+    n_det = int(o_npe.sum().item())
+    if int(counts[0]) != n_photons or int(counts[8]) != n_det:
+        raise RuntimeError(f"op counters inconsistent: {int(counts[0])} photons / {int(counts[8])} "
+                           f"detections counted, {n_photons} / {n_det} transported")
 
     ctx.set_timing(True)
     for _ in range(args.warmup):
@@ -151,6 +159,8 @@ def main() -> int:
             "frac": achieved / opmodel.VALU_PEAK_OPS,
             "traffic": None,
             "ops_per_launch": ops_per_launch,
+            "op_counts_per_photon": {k: round(int(v) / max(n_photons, 1), 4)
+                                     for k, v in zip(opmodel.COUNTER_NAMES, counts)},
             "fp32_nameplate_tflops": opmodel.FP32_NAMEPLATE_FLOPS / 1e12,
         },
         "cpu_baseline": None,
@@ -160,8 +170,42 @@ def main() -> int:
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["parity"] = _cpu_baseline(box, op, ro, dep, tau, args,
                                                            maps_gpu, npe_gpu)
+    if not args.no_surrogate:
+        out["surrogate"] = _surrogate(args, ctx)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _surrogate(args, ctx):
+    """Surrogate vs MC on CsI(Tl) (n 1.79, absorption 400 mm, Teflon R 0.98 over an air gap,
+    sigma_alpha 1.3 deg, 54k ph/MeV, decay 1000 ns, rise 20 ns): one network per crystal
+    thickness (10, 20, 30 mm) trained on uniform points x 1e4 photons, validated on held-out
+    points against MC means, then gamma events of 1-3 deposits in one multi-crystal queue
+    through the plain MC and through the surrogate with MC fallback within 1 mm of a face."""
+    from crysplight_amd import surrogate_eval as SE
+    from crysplight_amd.events import GammaMaterial
+    from crysplight_amd.params import Box, OpticalParams, SipmGrid, readout, deg2rad
+    op = OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.98, False, True, deg2rad(1.3), 0.3)
+    ro = readout(SipmGrid(3.2, 8, 8))
+    tau, rise, yld = 1000.0, 20.0, 54_000.0
+    thick = (10.0, 20.0, 30.0)
+    surs, train, val = [], {}, {}
+    for t in thick:
+        box = Box((25.6, 25.6, t))
+        sur, info = SE.train_for_box(box, op, ro, tau_ns=tau, tau_rise_ns=rise,
+                                     n_samples=args.surrogate_samples, photons=10_000, ctx=ctx,
+                                     seed=11 + int(t))
+        surs.append(sur)
+        train[f"{t:g}mm"] = info
+        val[f"{t:g}mm"] = SE.validate(sur, n_points=1000, photons=2_000_000, seed=77 + int(t),
+                                      ctx=ctx)
+    csi = GammaMaterial("xcom_CSI.csv", 4.51, "CsI")
+    dep, part_of, _, boxes = SE.gamma_mix(thick, 25.6, csi, yld, args.surrogate_events, seed=3)
+    cmp_ = SE.compare_events(surs, boxes, op, ro, dep, part_of, yield_per_mev=yld, tau_ns=tau,
+                             tau_rise_ns=rise, ctx=ctx)
+    return {"workload": "CsI(Tl) 25.6x25.6 mm, thicknesses 10/20/30 mm, Teflon, 54k ph/MeV; "
+                        "MLP 4-256-256-256-128 GELU, BF16 MFMA, FP32 Adam",
+            "training": train, "validation": val, "events": cmp_}
 
 
 def _traffic():

@@ -94,12 +94,16 @@ __device__ __forceinline__ void rng_init(Rng &r, uint64_t pid)
     r.has_next = 0u;
 }
 
+// Operation counters of the counting kernel variant, kept in the workgroup's LDS so they
+// add no registers: n_photon, n_draw, n_block, n_step, n_fresnel, n_surface_try,
+// n_lambert, n_rayleigh_try, n_detect, n_nscat.
 template <bool COUNT>
 struct OpCount {
-    // n_photon, n_draw, n_block, n_step, n_fresnel, n_surface_try, n_lambert,
-    // n_rayleigh_try, n_detect, n_nscat
-    uint32_t v[10];
-    __device__ __forceinline__ void add(int k, uint32_t d = 1u) { if (COUNT) v[k] += d; }
+    unsigned long long *lds;
+    __device__ __forceinline__ void add(int k, uint32_t d = 1u)
+    {
+        if (COUNT) atomicAdd(&lds[k], (unsigned long long)d);
+    }
 };
 
 // Top of a step: an empty current block takes the look-ahead one (counted as consumed),
@@ -396,16 +400,19 @@ struct Photon {
 
 // One iteration of the bounce loop (src/kernel.jl:185-298).  Returns the photon's
 // status after the step (ST_FLYING to continue); `lin` gets the detected SiPM index.
+// L0, L1, L2: the crystal's sides (the launch's box, or the photon's own box when
+// deposits of several crystals share one queue).
 template <bool COUNT>
 __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount<COUNT> &oc,
-                                    int &lin, int &nscat)
+                                    int &lin, int &nscat, const float L0, const float L1,
+                                    const float L2)
 {
     oc.add(3);
     // wall_hit (src/geometry.jl:19-32), ties x -> y -> z
     const float inf = __builtin_inff();
-    const float dx = p.ux > 0.0f ? (P.L0 - p.x) / p.ux : (p.ux < 0.0f ? -p.x / p.ux : inf);
-    const float dy = p.uy > 0.0f ? (P.L1 - p.y) / p.uy : (p.uy < 0.0f ? -p.y / p.uy : inf);
-    const float dz = p.uz > 0.0f ? (P.L2 - p.z) / p.uz : (p.uz < 0.0f ? -p.z / p.uz : inf);
+    const float dx = p.ux > 0.0f ? (L0 - p.x) / p.ux : (p.ux < 0.0f ? -p.x / p.ux : inf);
+    const float dy = p.uy > 0.0f ? (L1 - p.y) / p.uy : (p.uy < 0.0f ? -p.y / p.uy : inf);
+    const float dz = p.uz > 0.0f ? (L2 - p.z) / p.uz : (p.uz < 0.0f ? -p.z / p.uz : inf);
     float d;
     int face;
     if (dx <= dy && dx <= dz) { d = dx; face = p.ux > 0.0f ? F_XP : F_XM; }
@@ -426,13 +433,13 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         return p.bounces >= P.maxb ? ST_CAP : ST_FLYING;
     }
     // advance to the wall, snap onto it, clamp the rest (src/kernel.jl:206-218)
-    p.x = jclamp(p.x + d * p.ux, 0.0f, P.L0);
-    p.y = jclamp(p.y + d * p.uy, 0.0f, P.L1);
-    p.z = jclamp(p.z + d * p.uz, 0.0f, P.L2);
+    p.x = jclamp(p.x + d * p.ux, 0.0f, L0);
+    p.y = jclamp(p.y + d * p.uy, 0.0f, L1);
+    p.z = jclamp(p.z + d * p.uz, 0.0f, L2);
     const int a = face_axis(face);
-    if (a == 1)      p.x = face == F_XP ? P.L0 : 0.0f;
-    else if (a == 2) p.y = face == F_YP ? P.L1 : 0.0f;
-    else             p.z = face == F_BACK ? P.L2 : 0.0f;
+    if (a == 1)      p.x = face == F_XP ? L0 : 0.0f;
+    else if (a == 2) p.y = face == F_YP ? L1 : 0.0f;
+    else             p.z = face == F_BACK ? L2 : 0.0f;
     p.t += d * P.inv_v;
     const float cosi = a == 1 ? fabsf(p.ux) : (a == 2 ? fabsf(p.uy) : fabsf(p.uz));
 

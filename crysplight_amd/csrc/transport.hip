@@ -42,6 +42,8 @@ struct KBuffers {
     uint32_t *counts; uint32_t *tkeys; unsigned long long *counters;  // accumulate
     int32_t *maps; uint32_t *ev_tkeys;              // events
     unsigned long long *ev_tsum;                    // events, optional: sum of (t - t0) in 2^-10 ns
+    const float4 *boxes;                            // events, several crystals: (Lx, Ly, Lz, -)
+    const int32_t *dep_box;                         // events, several crystals: box of a deposit
     unsigned long long *cursor;                     // work-queue cursor (zeroed)
     unsigned long long *opcounts;                   // 10 op counters (COUNT only)
     unsigned int *flag;                             // events: input error flags
@@ -78,7 +80,7 @@ __device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep
     return lo;
 }
 
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool MULTIBOX = false>
 __global__ void __launch_bounds__(1024)
 transport_kernel(const KParams P, const KBuffers B)
 {
@@ -92,9 +94,12 @@ transport_kernel(const KParams P, const KBuffers B)
         __syncthreads();
     }
 
-    OpCount<COUNT> oc;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) oc.v[k] = 0u;
+    __shared__ unsigned long long lds_ops[10];
+    OpCount<COUNT> oc{lds_ops};
+    if (COUNT) {
+        if (threadIdx.x < 10) lds_ops[threadIdx.x] = 0ull;
+        __syncthreads();
+    }
 
     Photon p{};
     Rng rng;
@@ -102,6 +107,7 @@ transport_kernel(const KParams P, const KBuffers B)
     int64_t k = 0;                                  // photon number of this lane
     int32_t ev = 0;                                 // its event (event mode)
     float t_start = 0.0f;                           // its deposit time (event mode)
+    float bL0 = P.L0, bL1 = P.L1, bL2 = P.L2;       // its crystal
     int64_t my_dep = 0;                             // deposit of this lane's last photon
     int nscat = 0;
     bool alive = false;
@@ -160,6 +166,10 @@ transport_kernel(const KParams P, const KBuffers B)
                     x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
                     ev = B.dep_event[d];
                     t_start = t0;
+                    if (MULTIBOX) {
+                        const float4 bx = B.boxes[B.dep_box[d]];
+                        bL0 = bx.x; bL1 = bx.y; bL2 = bx.z;
+                    }
                 } else {
                     const int64_t j = P.npos == 1 ? 0 : k % (int64_t)P.npos;
                     x0 = B.px[j]; y0 = B.py[j]; z0 = B.pz[j];
@@ -179,7 +189,7 @@ transport_kernel(const KParams P, const KBuffers B)
         if (alive) {
             rng_prefetch<COUNT>(rng, P.key0, P.key1, oc);  // the step's Philox block, one site
             int lin = 0;
-            const int st = step<COUNT>(p, rng, P, oc, lin, nscat);
+            const int st = step<COUNT>(p, rng, P, oc, lin, nscat, bL0, bL1, bL2);
             if (st != ST_FLYING) {
                 alive = false;
                 if (MODE == M_RECORDS) {
@@ -222,11 +232,8 @@ transport_kernel(const KParams P, const KBuffers B)
     }
 
     if (COUNT) {
-#pragma unroll
-        for (int q = 0; q < 10; ++q) {
-            const unsigned long long s = wave_sum((unsigned long long)oc.v[q]);
-            if (__lane_id() == 0 && s) atomicAdd(&B.opcounts[q], s);
-        }
+        __syncthreads();
+        if (threadIdx.x < 10 && lds_ops[threadIdx.x]) atomicAdd(&B.opcounts[threadIdx.x], lds_ops[threadIdx.x]);
     }
     if (MODE == M_ACCUM) {
         __syncthreads();
@@ -312,7 +319,7 @@ __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids,
     P.key0 = (uint32_t)seed; P.key1 = (uint32_t)(seed >> 32);
     Rng r;
     rng_init(r, pid0 + (uint64_t)i + 1ull);
-    OpCount<false> oc;
+    OpCount<false> oc{nullptr};
     for (int d = 0; d < n_draws; ++d) out[i * n_draws + d] = rng_next<false>(r, P, oc);
 }
 
@@ -380,7 +387,7 @@ struct crysp_ctx {
 
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
-             B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_NBUF };
+             B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_BOXTAB, B_DEPBOX, B_NBUF };
 
 static int ensure(crysp_ctx *c, int id, size_t bytes, void **out)
 {
@@ -759,7 +766,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                        const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
                        uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
                        float *tmin, int32_t *npe, int64_t *n_out, bool use_graph,
-                       unsigned long long *tsum_dev = nullptr)
+                       unsigned long long *tsum_dev = nullptr, const crysp_box *boxes = nullptr,
+                       int32_t nbox = 0, const int32_t *dep_box = nullptr)
 {
     if (!c) return fail(CRYSP_EINVAL, "null context");
     if (ndep < 0 || nev < 0) return fail(CRYSP_EINVAL, "negative sizes");
@@ -812,6 +820,28 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     B.maps = (int32_t *)dmaps;
     B.ev_tkeys = (uint32_t *)dkeys;
     B.ev_tsum = tsum_dev;
+    const bool multibox = boxes != nullptr;
+    std::vector<float> boxtab;
+    if (multibox) {
+        if (nbox < 1 || nbox > 64 || !dep_box) return fail(CRYSP_EINVAL, "need 1..64 boxes and dep_box");
+        boxtab.resize((size_t)nbox * 4);
+        for (int b = 0; b < nbox; ++b) {
+            for (int a = 0; a < 3; ++a) {
+                if (!(boxes[b].L[a] > 0.0f) || !std::isfinite(boxes[b].L[a]))
+                    return fail(CRYSP_EINVAL, "box sides must be finite and > 0");
+                boxtab[4 * b + a] = boxes[b].L[a];
+            }
+            boxtab[4 * b + 3] = 0.0f;
+        }
+        if (!is_device_ptr(dep_box))
+            for (int64_t d = 0; d < ndep; ++d)
+                if (dep_box[d] < 0 || dep_box[d] >= nbox) return fail(CRYSP_EINVAL, "dep_box out of range");
+        void *dbt;
+        if ((rc = ensure(c, B_BOXTAB, (size_t)nbox * 16, &dbt))) return rc;
+        HIPCHK(hipMemcpyAsync(dbt, boxtab.data(), (size_t)nbox * 16, hipMemcpyHostToDevice, c->stream));
+        B.boxes = (const float4 *)dbt;
+        if ((rc = stage_in(c, B_DEPBOX, dep_box, (size_t)ndep * 4, (const void **)&B.dep_box))) return rc;
+    }
     void *cur;
     if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
     B.cursor = (unsigned long long *)cur;
@@ -856,7 +886,10 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
-            if (c->count_ops)
+            if (multibox)
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false, true>), dim3(grid), dim3(c->wg), 0,
+                                   c->stream, P, B);
+            else if (c->count_ops)
                 hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), 0,
                                    c->stream, P, B);
             else
@@ -926,6 +959,19 @@ int crysp_run_events(crysp_ctx *c, const crysp_box *box, const crysp_optical_par
 {
     return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
                        maxb, em, maps, tmin, npe, n_out, true);
+}
+
+int crysp_run_events_multibox(crysp_ctx *c, const crysp_box *boxes, int32_t nbox,
+                              const int32_t *dep_box, const crysp_optical_params *op,
+                              const crysp_readout *ro, const float *dep_xyzt,
+                              const int32_t *dep_nph, const int32_t *dep_event, int64_t ndep,
+                              int64_t nev, uint64_t seed, uint64_t pid0, int32_t maxb,
+                              const crysp_emission *em, int32_t *maps, float *tmin, int32_t *npe,
+                              int64_t *n_out)
+{
+    if (!boxes) return fail(CRYSP_EINVAL, "null boxes");
+    return events_impl(c, &boxes[0], op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
+                       maxb, em, maps, tmin, npe, n_out, true, nullptr, boxes, nbox, dep_box);
 }
 
 int crysp_light_maps(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,

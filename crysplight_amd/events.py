@@ -262,3 +262,50 @@ def autotune(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_n
         times.ctypes.data, C.byref(bw), C.byref(bp))
     _lib.check(rc, "crysp_autotune_events")
     return (bw.value, bp.value), times
+
+
+def transport_events_multibox(boxes, dep_box, op: OpticalParams, ro, dep: Deposits, *, seed: int,
+                              tau_ns: float, tau_rise_ns: float = 0.0, pid0: int = 0,
+                              max_bounces: int = 100_000, ctx=None) -> EventMaps:
+    """Event mode with deposits in several crystals in one queue: deposit d lies in
+    boxes[dep_box[d]]."""
+    ctx = ctx or _lib.default_context()
+    ro = readout(ro) if isinstance(ro, SipmGrid) else ro
+    nev = int(dep.n_events)
+    arr = (Box * len(boxes))(*[Box(tuple(b.L)) for b in boxes])
+    db = np.ascontiguousarray(dep_box, np.int32)
+    xyzt = np.ascontiguousarray(dep.xyzt, np.float32)
+    nph = np.ascontiguousarray(dep.nph, np.int32)
+    ev = np.ascontiguousarray(dep.event, np.int32)
+    maps = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.int32)
+    tmin = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.float32)
+    npe = np.zeros(nev, np.int32)
+    em = Emission(tau_ns, tau_rise_ns)
+    ntot = C.c_int64()
+    lib = _lib.load()
+    f = lib.crysp_run_events_multibox
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(OpticalParams),
+                  C.POINTER(Readout), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                  C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(Emission), C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_void_p]
+    rc = f(ctx.handle, C.addressof(arr), len(boxes), db.ctypes.data, C.byref(op), C.byref(ro),
+           xyzt.ctypes.data, nph.ctypes.data, ev.ctypes.data, xyzt.shape[0], nev, int(seed),
+           int(pid0), int(max_bounces), C.byref(em), maps.ctypes.data, tmin.ctypes.data,
+           npe.ctypes.data, C.byref(ntot))
+    _lib.check(rc, "crysp_run_events_multibox")
+    return EventMaps(maps, tmin, npe, ntot.value)
+
+
+def concat_deposits(parts) -> tuple[Deposits, np.ndarray, np.ndarray]:
+    """Concatenate deposit tables (events renumbered in order).  Returns the table, the
+    part index of every deposit, and the event offsets of every part."""
+    xyzt, nph, ev, ek, part = [], [], [], [], []
+    off = 0
+    ev_off = []
+    for i, d in enumerate(parts):
+        xyzt.append(d.xyzt); nph.append(d.nph); ev.append(d.event + off); ek.append(d.e_kev)
+        part.append(np.full(d.nph.shape[0], i, np.int32))
+        ev_off.append(off)
+        off += d.n_events
+    return (Deposits(np.concatenate(xyzt), np.concatenate(nph), np.concatenate(ev).astype(np.int32),
+                     np.concatenate(ek), off), np.concatenate(part), np.asarray(ev_off))

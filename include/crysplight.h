@@ -147,6 +147,17 @@ int crysp_run_events(crysp_ctx *ctx, const crysp_box *box,
                      const crysp_emission *em,
                      int32_t *maps, float *tmin, int32_t *npe, int64_t *n_photons_out);
 
+/* Event mode with deposits in several crystals sharing one queue (no reference
+ * counterpart): deposit d lies in crystal boxes[dep_box[d]] (nbox <= 64); everything
+ * else as crysp_run_events. */
+int crysp_run_events_multibox(crysp_ctx *ctx, const crysp_box *boxes, int32_t nbox,
+                              const int32_t *dep_box, const crysp_optical_params *op,
+                              const crysp_readout *ro, const float *dep_xyzt,
+                              const int32_t *dep_nph, const int32_t *dep_event, int64_t ndep,
+                              int64_t nev, uint64_t seed, uint64_t pid0, int32_t max_bounces,
+                              const crysp_emission *em, int32_t *maps, float *tmin, int32_t *npe,
+                              int64_t *n_photons_out);
+
 /* Light-response maps (no reference counterpart; the surrogate's Monte Carlo targets):
  * npts point sources at xyz[3i..3i+2] with nph[i] photons each (t0 = 0), one event per
  * point through the event mode.  counts: i32 (nx, ny, npts); tsum_fx (optional): u64

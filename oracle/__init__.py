@@ -248,7 +248,7 @@ def run_photons(box, op, ro, tb, **kw):
 
 def run_events(box, op, ro, dep_xyzt, dep_nph, dep_event, nev, *, seed, pid0=0,
                max_bounces=100_000, tau_ns=0.0, tau_rise_ns=0.0, threads=0,
-               return_records=False):
+               return_records=False, count_ops=False):
     dep = np.ascontiguousarray(np.asarray(dep_xyzt, np.float32).reshape(-1, 4))
     nph = np.ascontiguousarray(np.asarray(dep_nph, np.int32))
     dev = np.ascontiguousarray(np.asarray(dep_event, np.int32))
@@ -263,14 +263,17 @@ def run_events(box, op, ro, dep_xyzt, dep_nph, dep_event, nev, *, seed, pid0=0,
         rs, ri, rt = np.zeros(ntot, np.uint8), np.zeros(ntot, np.int16), np.zeros(ntot, np.float32)
     f = lib().or_run_events
     f.argtypes = [C.c_void_p] * 6 + [C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32,
-                                     C.c_float, C.c_float] + [C.c_void_p] * 6 + [C.c_int32]
+                                     C.c_float, C.c_float] + [C.c_void_p] * 7 + [C.c_int32]
     bx, o, r = _box(box), _op(op), _ro(ro)
+    oc = np.zeros(len(OPCOUNT_NAMES), np.int64) if count_ops else None
     f(C.byref(bx), C.byref(o), C.byref(r), _p(dep), _p(nph), _p(dev), dep.shape[0], nev, seed,
       pid0, max_bounces, tau_ns, tau_rise_ns, _p(maps), _p(tmin), _p(npe), _p(rs), _p(ri),
-      _p(rt), threads)
+      _p(rt), _p(oc), threads)
     out = {"maps": maps, "tmin": tmin, "npe": npe}
     if return_records:
         out.update(status=rs, idx=ri, t=rt)
+    if count_ops:
+        out["ops"] = dict(zip(OPCOUNT_NAMES, (int(v) for v in oc)))
     return out
 
 

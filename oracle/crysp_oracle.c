@@ -554,12 +554,14 @@ void or_reduce_records(const or_grid *grid, const or_tbin *tb, int64_t n,
  * deposit's (x, y, z) with t0 = deposit time.  dep_event[k] is the deposit's event.
  * Per-event reduction: src/events.jl:73-87 (maps summed over both faces via |idx|).
  * maps: i32 [nev*nx*ny] ((ii-1) + nx*(jj-1) + nx*ny*ev); tmin: f32 (+Inf = none);
- * npe: i32 [nev].  Records of every photon are also returned when the pointers are set. */
+ * npe: i32 [nev].  Records of every photon are also returned when the pointers are set;
+ * counts (optional) receives the summed operation counters. */
 void or_run_events(const or_box *box, const or_optical *op, const or_readout *ro,
                    const float *dep_xyzt, const int32_t *dep_nph, const int32_t *dep_event,
                    int64_t ndep, int64_t nev, uint64_t seed, uint64_t pid0, int32_t maxb,
                    float tau_d, float tau_r, int32_t *maps, float *tmin, int32_t *npe,
-                   uint8_t *rec_status, int16_t *rec_idx, float *rec_t, int32_t nthreads)
+                   uint8_t *rec_status, int16_t *rec_idx, float *rec_t, or_opcount *counts,
+                   int32_t nthreads)
 {
     int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ndep + 1));
     off[0] = 0;
@@ -568,22 +570,40 @@ void or_run_events(const or_box *box, const or_optical *op, const or_readout *ro
     uint8_t *st = rec_status ? rec_status : (uint8_t *)malloc((size_t)ntot + 1);
     int16_t *ix = rec_idx ? rec_idx : (int16_t *)malloc(sizeof(int16_t) * (size_t)(ntot + 1));
     float *tt = rec_t ? rec_t : (float *)malloc(sizeof(float) * (size_t)(ntot + 1));
+    or_opcount tot;
+    memset(&tot, 0, sizeof tot);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#pragma omp parallel num_threads(nthreads)
 #endif
-    for (int64_t k = 0; k < ndep; ++k) {
-        const float *d = dep_xyzt + 4 * k;
-        for (int64_t g = off[k]; g < off[k + 1]; ++g) {
-            or_stream s;
-            st_init(&s, seed, pid0 + (uint64_t)g + 1u, NULL);
-            float ux, uy, uz;
-            isotropic(&s, &ux, &uy, &uz);
-            float t = emission(&s, d[3], tau_d, tau_r);
-            or_rec r = propagate(&s, box, op, ro, d[0], d[1], d[2], ux, uy, uz, t, maxb);
-            st[g] = r.status; ix[g] = r.idx; tt[g] = r.t;
+    {
+        or_opcount loc;
+        memset(&loc, 0, sizeof loc);
+        or_opcount *ocp = counts ? &loc : NULL;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t k = 0; k < ndep; ++k) {
+            const float *d = dep_xyzt + 4 * k;
+            for (int64_t g = off[k]; g < off[k + 1]; ++g) {
+                or_stream s;
+                st_init(&s, seed, pid0 + (uint64_t)g + 1u, ocp);
+                float ux, uy, uz;
+                isotropic(&s, &ux, &uy, &uz);
+                float t = emission(&s, d[3], tau_d, tau_r);
+                or_rec r = propagate(&s, box, op, ro, d[0], d[1], d[2], ux, uy, uz, t, maxb);
+                st[g] = r.status; ix[g] = r.idx; tt[g] = r.t;
+                if (ocp) { loc.n_photon++; loc.n_nscat += r.nscat; }
+            }
+        }
+        if (counts) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            add_counts(&tot, &loc);
         }
     }
+    if (counts) *counts = tot;
     const int64_t nx = ro->grid.nx, nxy = (int64_t)ro->grid.nx * ro->grid.ny;
     memset(maps, 0, sizeof(int32_t) * (size_t)(nev * nxy));
     memset(npe, 0, sizeof(int32_t) * (size_t)nev);

@@ -70,6 +70,28 @@ def test_events_device_arrays_graph_replays(ctx, bgo_events):
     assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref2["maps"])
 
 
+def test_events_op_counts_match_oracle(ctx, bgo_events):
+    """The counting kernel variant in event mode, eager (host arrays) and through the
+    captured graph (device arrays, twice), against the oracle's counters of the same
+    photons."""
+    import torch
+    dep, _ = bgo_events
+    ref = oracle.run_events(K.C2_BOX, K.C2_OP, RO8, dep.xyzt, dep.nph, dep.event, dep.n_events,
+                            seed=7, tau_ns=K.C2_TAU, count_ops=True)
+    want = [ref["ops"][n] for n in oracle.OPCOUNT_NAMES]
+    assert want[0] == dep.n_photons and want[8] == int(ref["npe"].sum())
+    dev = torch.device("cuda", 0)
+    darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event))
+    ctx.set_op_counting(True)
+    try:
+        for rep, kw in enumerate([{}, {"device_arrays": darr}, {"device_arrays": darr}]):
+            ctx.op_counts(reset=True)
+            E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx, **kw)
+            assert list(ctx.op_counts(reset=True)) == want, rep
+    finally:
+        ctx.set_op_counting(False)
+
+
 def test_events_rise_time_uneven(ctx):
     """Batched uneven single-deposit events with rise + decay emission."""
     dep = E.synthetic_deposits(K.C1_BOX, n_events=300, seed=11, e_kev=(10.0, 511.0),

@@ -330,18 +330,25 @@ def test_event_reduction_matches_records():
     ev = np.sort(rng.integers(0, 15, ndep)).astype(np.int32)
     ro2 = readout(K.GRID, two_sided=True)
     r = oracle.run_events(K.BOX_CRYSP, K.OP_ALL, ro2, dep, nph, ev, 15, seed=3, pid0=17,
-                          tau_ns=800.0, return_records=True)
+                          tau_ns=800.0, return_records=True, count_ops=True)
     # recompute with the records mode of the same photons, one deposit at a time
     g0 = 0
     maps = np.zeros((15, 64), np.int64)
+    ops = dict.fromkeys(oracle.OPCOUNT_NAMES, 0)
     for d in range(ndep):
         rec = oracle.transport_records(K.BOX_CRYSP, K.OP_ALL, ro2, n_photons=int(nph[d]),
                                        seed=3, pos=tuple(dep[d, :3]), t0s=[dep[d, 3]],
-                                       pid_offset=17 + g0, tau_ns=800.0)
+                                       pid_offset=17 + g0, tau_ns=800.0, count_ops=True)
         det = rec["status"] == STATUS_DETECTED
         np.add.at(maps[ev[d]], np.abs(rec["idx"][det].astype(np.int64)) - 1, 1)
         assert np.array_equal(rec["status"], r["status"][g0:g0 + nph[d]])
+        for k in ops:
+            ops[k] += rec["ops"][k]
         g0 += int(nph[d])
     assert np.array_equal(maps, r["maps"])
+    # the event mode's operation counters cover exactly the same photons
+    assert r["ops"] == ops
+    assert r["ops"]["n_photon"] == int(nph.sum())
+    assert r["ops"]["n_detect"] == int(r["npe"].sum())
     assert np.array_equal(r["npe"], maps.sum(1))
     assert np.array_equal(np.isfinite(r["tmin"]), r["maps"] > 0)

@@ -106,3 +106,30 @@ def test_training_reduces_loss_on_a_smooth_target():
     for _ in range(30):
         last = net.train_batches(Xd, Td, None, 4, 4096, lr=2e-3)
     assert last.mean() < 0.1 * first[0], (first, last)
+
+
+@gpu
+def test_surrogate_end_to_end_small():
+    """A small surrogate trains, beats its untrained start by a wide margin on held-out MC
+    means, and its event maps (with MC fallback) keep the energy sum within a few percent
+    of the plain MC while running faster."""
+    from crysplight_amd import surrogate_eval as SE
+    from crysplight_amd.events import GammaMaterial
+    from crysplight_amd.params import Box
+    from tests import cases as K
+    box = Box((25.6, 25.6, 20.0))
+    sur0, _ = SE.train_for_box(box, K.C1_OP, K.C1_RO, tau_ns=1000.0, n_samples=8192,
+                               photons=2000, epochs=1, batch=8192)
+    err0 = SE.validate(sur0, n_points=100, photons=400_000)["count_error_beyond_fallback"]
+    sur, info = SE.train_for_box(box, K.C1_OP, K.C1_RO, tau_ns=1000.0, n_samples=100_000,
+                                 photons=4000, epochs=15, batch=4096)
+    v = SE.validate(sur, n_points=100, photons=400_000)
+    assert v["count_error_beyond_fallback"] < 0.12, v
+    assert v["count_error_beyond_fallback"] < 0.5 * err0, (err0, v)
+    csi = GammaMaterial("xcom_CSI.csv", 4.51)
+    dep, part_of, _, boxes = SE.gamma_mix((20.0,), 25.6, csi, 54_000.0, 3000, seed=3)
+    assert dep.n_events == 3000
+    r = SE.compare_events([sur], boxes, K.C1_OP, K.C1_RO, dep, part_of, yield_per_mev=54_000.0,
+                          tau_ns=1000.0)
+    assert abs(r["energy_sum_bias"]) < 0.05, r
+    assert r["speedup_sampling_only"] > 10, r
