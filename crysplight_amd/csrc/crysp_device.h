@@ -50,9 +50,13 @@ struct KParams {
 
 // ------------------------------------------------------------------ Philox ----
 // Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
+// The key is uniform: the empty asm makes it opaque here, so the ten bumped keys are
+// scalar adds at each use instead of twenty precomputed values held (and spilled)
+// across the whole kernel.
 __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                          uint32_t k0, uint32_t k1)
 {
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         // one 32x32->64 product each (v_mad_u64_u32) gives both halves
@@ -68,30 +72,36 @@ __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
     return make_uint4(c0, c1, c2, c3);
 }
 
-// Out-of-line copy for the rare refill in the middle of a step.
-__device__ __noinline__ uint4 philox10_call(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t k0,
-                                            uint32_t k1)
-{
-    return philox10(c0, c1, c2, 0u, k0, k1);
-}
-
 // Per-photon stream: counter (pid lo, pid hi, block, 0); four words per block, used in
-// order (src/philox.jl:34-57).  The current block shifts down as it is consumed; a
-// look-ahead block is generated once per bounce step at a single site for all lanes
-// (rng_prefetch), so the draws inside a step rarely need Philox in a divergent branch.
+// order (src/philox.jl:34-57).  Generated words wait in a ring of 16 words per lane in
+// LDS: 16 slots x 64 lanes per wavefront, slot-major, so a wavefront's reads and writes
+// are free of bank conflicts whatever slot each lane is at.  A new photon starts with
+// blocks 0 and 1 (rng_start); after that one block is generated per bounce step at a
+// single site for all lanes while the ring has room (rng_prefetch), so a step starts
+// with 8..16 words in hand and a draw is one LDS read.  Only a step that outruns the
+// ring generates a block at the draw itself.
+constexpr uint32_t RING_SLOT = 256u;              // bytes between a lane's ring slots
+constexpr uint32_t RING_POS_MASK = 15u * RING_SLOT;
+constexpr uint32_t RING_WAVE_BYTES = 16u * RING_SLOT;
+
 struct Rng {
-    uint32_t lo, hi, blk, left;               // blk: index of the next block to generate
-    uint32_t w0, w1, w2, w3;                  // current block
-    uint32_t n0, n1, n2, n3;                  // look-ahead block, valid when has_next
-    uint32_t has_next;
+    uint32_t lo, hi, blk;                     // counter words of the next block to generate
+    uint32_t rd, wr;                          // words drawn / generated, x RING_SLOT
+    uint32_t base;                            // this lane's byte offset in ring slot 0
+    char *ring;                               // the workgroup's rings (LDS)
 };
+
+// ring: LDS of blockDim.x * 64 bytes; blockDim.x is a multiple of 64
+__device__ __forceinline__ void rng_bind(Rng &r, void *ring)
+{
+    r.ring = (char *)ring;
+    r.base = (threadIdx.x >> 6) * RING_WAVE_BYTES + (threadIdx.x & 63u) * 4u;
+}
 
 __device__ __forceinline__ void rng_init(Rng &r, uint64_t pid)
 {
-    r.lo = (uint32_t)pid; r.hi = (uint32_t)(pid >> 32); r.blk = 0u; r.left = 0u;
-    r.w0 = r.w1 = r.w2 = r.w3 = 0u;
-    r.n0 = r.n1 = r.n2 = r.n3 = 0u;
-    r.has_next = 0u;
+    r.lo = (uint32_t)pid; r.hi = (uint32_t)(pid >> 32); r.blk = 0u;
+    r.rd = 0u; r.wr = 0u;
 }
 
 // Operation counters of the counting kernel variant, kept in the workgroup's LDS so they
@@ -106,44 +116,42 @@ struct OpCount {
     }
 };
 
-// Top of a step: an empty current block takes the look-ahead one (counted as consumed),
-// then a missing look-ahead block is generated, so a step starts with 5..8 words.
-template <bool COUNT>
-__device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1, OpCount<COUNT> &oc)
+// Generate the next block into the ring (the caller guarantees four free slots).  The
+// writer's slot is a multiple of four, so the block's slots are consecutive.
+__device__ __forceinline__ void rng_fill(Rng &r, uint32_t k0, uint32_t k1)
 {
-    if (r.left == 0u && r.has_next) {
-        r.w0 = r.n0; r.w1 = r.n1; r.w2 = r.n2; r.w3 = r.n3;
-        r.has_next = 0u;
-        r.left = 4u;
-        oc.add(2);
-    }
-    if (!r.has_next) {
-        const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, k0, k1);
-        r.n0 = b.x; r.n1 = b.y; r.n2 = b.z; r.n3 = b.w;
-        r.blk += 1u;
-        r.has_next = 1u;
-    }
+    const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, k0, k1);
+    char *q = r.ring + ((r.wr & RING_POS_MASK) | r.base);
+    *(uint32_t *)(q) = b.x;
+    *(uint32_t *)(q + RING_SLOT) = b.y;
+    *(uint32_t *)(q + 2 * RING_SLOT) = b.z;
+    *(uint32_t *)(q + 3 * RING_SLOT) = b.w;
+    r.blk += 1u;
+    r.wr += 4u * RING_SLOT;
+}
+
+// A new photon: blocks 0 and 1.
+__device__ __forceinline__ void rng_start(Rng &r, uint64_t pid, uint32_t k0, uint32_t k1)
+{
+    rng_init(r, pid);
+    rng_fill(r, k0, k1);
+    rng_fill(r, k0, k1);
+}
+
+// Top of a step: one block while the ring has four free slots.
+__device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1)
+{
+    if (r.wr - r.rd <= 12u * RING_SLOT) rng_fill(r, k0, k1);
 }
 
 template <bool COUNT>
 __device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<COUNT> &oc)
 {
-    if (r.left == 0u) {
-        if (r.has_next) {
-            r.w0 = r.n0; r.w1 = r.n1; r.w2 = r.n2; r.w3 = r.n3;
-            r.has_next = 0u;
-        } else {
-            const uint4 b = philox10_call(r.lo, r.hi, r.blk, P.key0, P.key1);
-            r.w0 = b.x; r.w1 = b.y; r.w2 = b.z; r.w3 = b.w;
-            r.blk += 1u;
-        }
-        r.left = 4u;
-        oc.add(2);                            // blocks counted when consumed
-    }
+    if (r.rd == r.wr) rng_fill(r, P.key0, P.key1);
+    if (COUNT && ((r.rd / RING_SLOT) & 3u) == 0u) oc.add(2);   // a block's first word
     oc.add(1);
-    const uint32_t x = r.w0;
-    r.w0 = r.w1; r.w1 = r.w2; r.w2 = r.w3;
-    r.left -= 1u;
+    const uint32_t x = *(const uint32_t *)(r.ring + ((r.rd & RING_POS_MASK) | r.base));
+    r.rd += RING_SLOT;
     return x;
 }
 

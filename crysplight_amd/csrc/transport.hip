@@ -84,7 +84,10 @@ template <int MODE, bool COUNT, bool MULTIBOX = false>
 __global__ void __launch_bounds__(1024)
 transport_kernel(const KParams P, const KBuffers B)
 {
-    extern __shared__ uint32_t lds[];               // accumulate: hist + time keys
+    // dynamic LDS: the lanes' Philox rings (blockDim.x * 64 B), then in accumulate mode
+    // the histogram and time keys
+    extern __shared__ uint32_t lds_dyn[];
+    uint32_t *lds = lds_dyn + 16u * blockDim.x;
     __shared__ unsigned long long lds_ctr[9];       // accumulate: the Accumulator counters
     const int nxy = P.nx * P.ny;
     if (MODE == M_ACCUM) {
@@ -103,6 +106,7 @@ transport_kernel(const KParams P, const KBuffers B)
 
     Photon p{};
     Rng rng;
+    rng_bind(rng, lds_dyn);
     rng_init(rng, 0);
     int64_t k = 0;                                  // photon number of this lane
     int32_t ev = 0;                                 // its event (event mode)
@@ -175,7 +179,7 @@ transport_kernel(const KParams P, const KBuffers B)
                     x0 = B.px[j]; y0 = B.py[j]; z0 = B.pz[j];
                     t0 = B.pt ? B.pt[j] : 0.0f;
                 }
-                rng_init(rng, P.pid0 + (uint64_t)k + 1ull);
+                rng_start(rng, P.pid0 + (uint64_t)k + 1ull, P.key0, P.key1);
                 p.x = x0; p.y = y0; p.z = z0; p.bounces = 0;
                 emit<COUNT>(rng, P, oc, t0, p.ux, p.uy, p.uz, p.t);
                 nscat = 0;
@@ -187,7 +191,7 @@ transport_kernel(const KParams P, const KBuffers B)
 
         // ---- one bounce step ------------------------------------------------------
         if (alive) {
-            rng_prefetch<COUNT>(rng, P.key0, P.key1, oc);  // the step's Philox block, one site
+            rng_prefetch(rng, P.key0, P.key1);      // the step's Philox block, one site
             int lin = 0;
             const int st = step<COUNT>(p, rng, P, oc, lin, nscat, bL0, bL1, bL2);
             if (st != ST_FLYING) {
@@ -315,9 +319,11 @@ __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids,
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_pids) return;
+    extern __shared__ uint32_t ring[];
     KParams P{};
     P.key0 = (uint32_t)seed; P.key1 = (uint32_t)(seed >> 32);
     Rng r;
+    rng_bind(r, ring);
     rng_init(r, pid0 + (uint64_t)i + 1ull);
     OpCount<false> oc{nullptr};
     for (int d = 0; d < n_draws; ++d) out[i * n_draws + d] = rng_next<false>(r, P, oc);
@@ -485,6 +491,11 @@ static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_par
     return CRYSP_OK;
 }
 
+// LDS of the lanes' Philox rings: 16 words per lane
+static size_t ring_bytes(int wg) { return (size_t)wg * 64; }
+// the transport kernel's static LDS (counters) stays below this
+constexpr size_t STATIC_LDS_MARGIN = 256;
+
 static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
 {
     // persistent grid: ceil(n / (wg*ppt)) workgroups, at most what stays resident
@@ -499,6 +510,7 @@ template <int MODE>
 static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_bytes)
 {
     const int wg = c->wg;
+    lds_bytes += ring_bytes(wg);
     const int grid = grid_for(c, P.n_photons, wg, c->ppt, lds_bytes);
     P.chunk = c->chunk;
     P.refill_min = c->refill_min;
@@ -602,6 +614,10 @@ int crysp_set_schedule(crysp_ctx *c, int32_t wg, int32_t ppt)
 {
     if (!c) return fail(CRYSP_EINVAL, "null context");
     if (wg < 64 || wg > 1024 || wg % 64) return fail(CRYSP_EINVAL, "wg_size must be 64..1024, multiple of 64");
+    if (ring_bytes(wg) + STATIC_LDS_MARGIN > c->lds_per_block)
+        return fail(CRYSP_ERANGE, "wg_size " + std::to_string(wg) + " needs " +
+                                  std::to_string(ring_bytes(wg)) + " B of LDS for the Philox rings; "
+                                  "the device allows " + std::to_string(c->lds_per_block) + " B per workgroup");
     if (ppt < 1 || ppt > 64) return fail(CRYSP_EINVAL, "photons_per_thread must be 1..64");
     c->wg = wg;
     c->ppt = ppt;
@@ -716,8 +732,9 @@ int crysp_transport_accumulate(crysp_ctx *c, const crysp_box *box,
     const int64_t nxy = (int64_t)ro->grid.nx * ro->grid.ny;
     const int64_t hist = nxy * ((int64_t)tb->nbins + 1);
     const size_t lds_bytes = (size_t)(hist + nxy) * 4;
-    if (lds_bytes > 64 * 1024)
-        return fail(CRYSP_ERANGE, "nx*ny*(nbins+2) words exceed the 64 KiB workgroup LDS histogram");
+    if (lds_bytes + ring_bytes(c->wg) + STATIC_LDS_MARGIN > c->lds_per_block)
+        return fail(CRYSP_ERANGE, "nx*ny*(nbins+2) histogram words plus 16 words per lane exceed "
+                                  "the workgroup LDS (" + std::to_string(c->lds_per_block) + " B)");
     P.n_photons = n;
     P.npos = npos;
     P.nbins = tb->nbins;
@@ -855,7 +872,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     // the grid only sizes the persistent launch; the kernel drains whatever total it reads
     const int64_t n_hint = ntot_host >= 0 ? ntot_host : (c->last_n_hint > 0 ? c->last_n_hint : (int64_t)1 << 40);
     P.n_photons = -1;                               // read dep_off[ndep] on the device
-    const int grid = grid_for(c, n_hint, c->wg, c->ppt, 0);
+    const size_t lds_ev = ring_bytes(c->wg);
+    const int grid = grid_for(c, n_hint, c->wg, c->ppt, lds_ev);
 
     std::vector<unsigned char> key(sizeof P + sizeof B + sizeof(void *) + 6 * sizeof(int64_t));
     {
@@ -887,13 +905,13 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             if (multibox)
-                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false, true>), dim3(grid), dim3(c->wg), 0,
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false, true>), dim3(grid), dim3(c->wg), lds_ev,
                                    c->stream, P, B);
             else if (c->count_ops)
-                hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), 0,
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), lds_ev,
                                    c->stream, P, B);
             else
-                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg), 0,
+                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg), lds_ev,
                                    c->stream, P, B);
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev1, c->stream));
         }
@@ -1035,6 +1053,11 @@ int crysp_autotune_events(crysp_ctx *c, const crysp_box *box,
     c->timing = true;
     for (int a = 0; a < n_wg; ++a) {
         for (int b = 0; b < n_ppt; ++b) {
+            if (wgs[a] >= 64 && wgs[a] <= 1024 &&
+                ring_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
+                if (times_ms) times_ms[a * n_ppt + b] = INFINITY;   // does not fit: skipped
+                continue;
+            }
             if ((rc = crysp_set_schedule(c, wgs[a], ppts[b]))) goto done;
             float tbest = 1e30f;
             for (int r = 0; r < reps; ++r) {
@@ -1071,7 +1094,7 @@ int crysp_debug_draws(crysp_ctx *c, uint64_t seed, uint64_t pid0, int64_t n_pids
     void *d;
     int rc = out_buf(c, B_MISC, out, (size_t)n_pids * n_draws * 4, &d);
     if (rc) return rc;
-    hipLaunchKernelGGL(debug_draws_kernel, dim3((unsigned)((n_pids + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(debug_draws_kernel, dim3((unsigned)((n_pids + 255) / 256)), dim3(256), 256 * 64,
                        c->stream, seed, pid0, n_pids, n_draws, (uint32_t *)d);
     HIPCHK(hipGetLastError());
     if ((rc = stage_out(c, out, d, (size_t)n_pids * n_draws * 4))) return rc;
