@@ -30,6 +30,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = "optical photons/s on 1 GPU; surrogate vs MC events/s at ≤2% map error"
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def workload():
     from crysplight_amd.params import Box, OpticalParams, SipmGrid, readout, deg2rad
     box = Box((25.6, 25.6, 20.0))
@@ -52,6 +56,8 @@ def main() -> int:
                     help="skip the surrogate training / validation / event comparison")
     ap.add_argument("--surrogate-events", type=int, default=1_000_000)
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000)
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the point-source and batched-uneven-events lines")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this size once (a plain copy "
                          "kernel of known traffic, to scale rocprofv3 FETCH/WRITE_SIZE)")
@@ -98,6 +104,7 @@ def main() -> int:
     ctx.set_op_counting(False)
     ops_per_launch = opmodel.algorithmic_ops(counts)
     n_photons = r.n_photons
+    _progress(f"op-count pass: {ops_per_launch / n_photons:.1f} ops per photon")
     n_det = int(o_npe.sum().item())
     if int(counts[0]) != n_photons or int(counts[8]) != n_det:
         raise RuntimeError(f"op counters inconsistent: {int(counts[0])} photons / {int(counts[8])} "
@@ -115,6 +122,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     kern_ms, n_launch = ctx.kernel_time(reset=True)
     avg_kernel_s = kern_ms / max(n_launch, 1) / 1e3
+    _progress(f"timed {args.steps} steps: {avg_kernel_s * 1e3:.3f} ms per launch")
 
     if args.copy_calibration_mib:
         n = args.copy_calibration_mib * (1 << 20) // 4
@@ -125,8 +133,17 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    # the same step from host arrays: deposits in and maps out over PCIe
+    ctx.set_timing(False)
+    t0 = time.perf_counter()
+    rh = E.transport_events(box, op, ro, dep, seed=args.seed, tau_ns=tau, ctx=ctx)
+    host_s = time.perf_counter() - t0
+
     maps_gpu = o_maps.cpu().numpy()
     npe_gpu = o_npe.cpu().numpy()
+    if not (np.array_equal(rh.maps.reshape(nev, -1), maps_gpu.reshape(nev, -1))
+            and np.array_equal(rh.npe, npe_gpu)):
+        raise RuntimeError("host-array and device-array runs of the same step differ")
     value = n_photons * args.steps / elapsed
     achieved = ops_per_launch / avg_kernel_s
     out = {
@@ -151,6 +168,7 @@ def main() -> int:
             "parallelism": "single GPU, one persistent launch per step (hipGraph)",
         },
         "kernel_ms": avg_kernel_s * 1e3,
+        "end_to_end_host_arrays_photons_per_s": n_photons / host_s,
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
@@ -170,10 +188,80 @@ def main() -> int:
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["parity"] = _cpu_baseline(box, op, ro, dep, tau, args,
                                                            maps_gpu, npe_gpu)
+        _progress(f"cpu baseline: {out['cpu_baseline']['value']:.3g} photons/s, parity {out['parity']}")
+    if not args.no_other_configs:
+        out["other_configs"] = _other_configs(ctx)
+        _progress("point-source and batched-uneven lines done")
     if not args.no_surrogate:
         out["surrogate"] = _surrogate(args, ctx)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _other_configs(ctx):
+    """Point source (1e6 photons, accumulate mode, CsI(Tl)/Teflon) and batched uneven
+    events (1e5 single-deposit CsI(Tl) events, 54k ph/MeV, rise 20 ns, decay 1 us, one
+    persistent launch per step with the autotuned schedule).  Kernel rates from HIP events."""
+    import numpy as np
+    import torch
+    from crysplight_amd.params import Box, OpticalParams, SipmGrid, TimeBinning, readout, deg2rad
+    from crysplight_amd import transport as T
+    from crysplight_amd import events as E
+
+    box = Box((25.6, 25.6, 20.0))
+    op = OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.98, False, True, deg2rad(1.3), 0.3)
+    ro = readout(SipmGrid(3.2, 8, 8))
+    tb = TimeBinning(100.0, 80)
+    res = {}
+
+    n1 = 1_000_000
+    kw = dict(n_photons=n1, seed=1234, pos=(12.8, 12.8, 10.0), tau_ns=1000.0, ctx=ctx)
+    T.transport_accumulate(box, op, ro, tb, **kw)
+    ctx.set_timing(True)
+    ctx.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        acc = T.transport_accumulate(box, op, ro, tb, **kw)
+    e2e = (time.perf_counter() - t0) / 5
+    ms, n = ctx.kernel_time(reset=True)
+    res["point_source_csitl"] = {
+        "photons": n1, "kernel_ms": ms / n, "kernel_photons_per_s": n1 / (ms / n / 1e3),
+        "end_to_end_photons_per_s": n1 / e2e, "detected_fraction": acc.ndet / n1}
+
+    dep = E.synthetic_deposits(box, n_events=100_000, seed=11, e_kev=(10.0, 511.0),
+                               yield_per_mev=54_000.0)
+    m = 10_000
+    sub = E.Deposits(dep.xyzt[:m], dep.nph[:m], dep.event[:m], dep.e_kev[:m], m)
+    ctx.set_timing(False)
+    (wg, ppt), times = E.autotune(box, op, ro, sub, seed=5, tau_ns=1000.0, tau_rise_ns=20.0,
+                                  ctx=ctx, ppts=(1, 2, 4, 8))
+    dev = torch.device("cuda", 0)
+    darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event))
+    nev = dep.n_events
+    out = (torch.zeros((nev, 8, 8), dtype=torch.int32, device=dev),
+           torch.zeros((nev, 8, 8), dtype=torch.float32, device=dev),
+           torch.zeros(nev, dtype=torch.int32, device=dev))
+    kw3 = dict(seed=5, tau_ns=1000.0, tau_rise_ns=20.0, ctx=ctx, device_arrays=darr, out=out)
+    r = E.transport_events(box, op, ro, dep, **kw3)
+    ctx.set_timing(True)
+    ctx.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        r = E.transport_events(box, op, ro, dep, **kw3)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / 3
+    ms, n = ctx.kernel_time(reset=True)
+    ctx.set_timing(False)
+    nph = dep.nph.astype(np.int64)
+    res["batched_uneven_csitl"] = {
+        "events": nev, "photons_per_step": r.n_photons,
+        "photons_per_event_min_max": [int(nph.min()), int(nph.max())],
+        "autotuned_schedule": {"wg": wg, "photons_per_thread": ppt,
+                               "sweep_ms_min_max": [float(np.min(times[np.isfinite(times)])),
+                                                    float(np.max(times[np.isfinite(times)]))]},
+        "kernel_ms": ms / n, "kernel_photons_per_s": r.n_photons / (ms / n / 1e3),
+        "events_per_s": nev / wall, "step_ms": wall * 1e3}
+    return res
 
 
 def _surrogate(args, ctx):
@@ -199,6 +287,7 @@ def _surrogate(args, ctx):
         train[f"{t:g}mm"] = info
         val[f"{t:g}mm"] = SE.validate(sur, n_points=1000, photons=2_000_000, seed=77 + int(t),
                                       ctx=ctx)
+        _progress(f"surrogate {t:g} mm trained and validated")
     csi = GammaMaterial("xcom_CSI.csv", 4.51, "CsI")
     dep, part_of, _, boxes = SE.gamma_mix(thick, 25.6, csi, yld, args.surrogate_events, seed=3)
     cmp_ = SE.compare_events(surs, boxes, op, ro, dep, part_of, yield_per_mev=yld, tau_ns=tau,

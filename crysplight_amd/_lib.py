@@ -13,7 +13,8 @@ from pathlib import Path
 from . import params as P
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "libcrysplight_hip.so"
+# CRYSPLIGHT_HIP_LIB names another build of the library (the Julia binding reads the same)
+LIB_PATH = Path(os.environ.get("CRYSPLIGHT_HIP_LIB", HERE / "libcrysplight_hip.so"))
 HEADER = HERE.parent / "include" / "crysplight.h"
 
 _lib = None

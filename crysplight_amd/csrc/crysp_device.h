@@ -186,7 +186,16 @@ __device__ __forceinline__ float f_log(float x)
     uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u;
     if (mb > 0x3FB504F3u) { mb -= 0x00800000u; ++e; }      // m >= sqrt(2): halve it
     const double m = (double)__uint_as_float(mb);
-    const double s = (m - 1.0) / (m + 1.0);                  // m-1, m+1 exact
+    // s = (m-1)/(m+1) (m-1, m+1 exact) from the hardware reciprocal and two Newton steps:
+    // ~1 ulp of binary64 instead of a correctly rounded division, well inside what the
+    // single rounding to binary32 needs
+    const double num = m - 1.0, den = m + 1.0;
+    double rc = __builtin_amdgcn_rcp(den);
+    double er = __builtin_fma(-den, rc, 1.0);
+    rc = __builtin_fma(rc, er, rc);
+    er = __builtin_fma(-den, rc, 1.0);
+    rc = __builtin_fma(rc, er, rc);
+    const double s = num * rc;
     const double z = s * s;
     double p = 0.043478260869565216;                         // 1/23
     p = __builtin_fma(p, z, 0.047619047619047616);           // 1/21

@@ -219,7 +219,7 @@ int crysp_get_op_counts(crysp_ctx *ctx, int64_t *counts10, int32_t reset);
 
 /* ---- host-side event feed (CPU code in the same library) ------------------------
  * The gamma stage that produces deposits for crysp_run_events: the reference's vendored
- * gamma core (src/gamma/*.jl) and its bridge (src/gamma_interface.jl), Float64 cm/MeV.
+ * gamma core (the .jl files of src/gamma/) and its bridge (src/gamma_interface.jl), Float64 cm/MeV.
  * Random numbers come from per-event Philox streams (the reference's Xoshiro cannot be
  * reproduced), so events agree with the reference statistically, not bit for bit. */
 typedef struct crysp_gamma_material crysp_gamma_material;
