@@ -192,6 +192,62 @@ def mc_light_maps(box: Box, op: OpticalParams, ro, xyz, nph, *, seed: int, tau_n
     return counts, tsum.astype(np.float64) / 1024.0
 
 
+# ---------------------------------------------------------------- symmetry ----
+# A square crystal face read by a square SiPM grid that spans it exactly is symmetric
+# under x -> L-x, y -> L-y and x <-> y (8 operations).  SiPM (i, j) (0-based, i along x)
+# is cell i + n*j.  A position folds into the fundamental triangle 0 <= x <= y <= L/2
+# (octant code: bit 0 x mirrored, bit 1 y mirrored, bit 2 swapped); the network learns
+# maps of folded positions only, in the folded cell order.
+
+def fold_xy(x, y, L):
+    """(x, y) -> folded (xc, yc) and the octant code of each point."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    h = 0.5 * L
+    mx, my = x > h, y > h
+    xf = np.where(mx, L - x, x)
+    yf = np.where(my, L - y, y)
+    sw = xf > yf
+    xc = np.where(sw, yf, xf)
+    yc = np.where(sw, xf, yf)
+    code = mx.astype(np.int64) | (my.astype(np.int64) << 1) | (sw.astype(np.int64) << 2)
+    return xc, yc, code
+
+
+def fold_perms(n: int) -> np.ndarray:
+    """[8, n*n]: for octant code o, perms[o][c'] is the actual cell of folded cell c'."""
+    a1, b1 = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")   # a1 along x
+    a1, b1 = a1.ravel(), b1.ravel()                                   # c' = a1 + n*b1
+    out = np.empty((8, n * n), np.int64)
+    for o in range(8):
+        mx, my, sw = o & 1, (o >> 1) & 1, (o >> 2) & 1
+        a, b = (b1, a1) if sw else (a1, b1)
+        i = n - 1 - a if mx else a
+        j = n - 1 - b if my else b
+        out[o] = i + n * j
+    return out
+
+
+def to_folded(maps, code, perms):
+    """Actual-order maps [k, n*n] -> folded order."""
+    return np.take_along_axis(np.asarray(maps), perms[code], axis=1)
+
+
+def from_folded(maps_f, code, perms):
+    """Folded-order maps [k, n*n] -> actual order."""
+    out = np.empty_like(maps_f)
+    np.put_along_axis(out, perms[code], maps_f, axis=1)
+    return out
+
+
+def folds(box: Box, ro) -> bool:
+    """True when the optics have the 8-fold symmetry the folding uses."""
+    g = ro.grid
+    L0, L1 = float(box.L[0]), float(box.L[1])
+    return (abs(L0 - L1) <= 1e-6 * L0 and g.nx == g.ny and not ro.disc
+            and abs(g.pitch_mm * g.nx - L0) <= 1e-5 * L0)
+
+
 @dataclass
 class TrainingSet:
     X: np.ndarray        # (n, 4) x, y, z [mm], E [keV]
@@ -214,10 +270,13 @@ class LightSurrogate:
 
     def __init__(self, box: Box, op: OpticalParams, ro, *, tau_ns: float,
                  tau_rise_ns: float = 0.0, e_range=(10.0, 511.0), hidden=256, n_hidden_layers=3,
-                 seed=1, device=0, log_depth_mm: float = 0.0):
+                 seed=1, device=0, log_depth_mm: float = 0.0, fold: bool = True):
         self.box, self.op = box, op
         self.ro = readout(ro) if isinstance(ro, SipmGrid) else ro
         self.ncell = self.ro.grid.nx * self.ro.grid.ny
+        # fold positions into 1/8 of the face (and permute maps) when the optics allow it
+        self.fold = bool(fold) and folds(box, self.ro)
+        self.perms = fold_perms(self.ro.grid.nx) if self.fold else None
         self.tau_ns, self.tau_rise_ns = float(tau_ns), float(tau_rise_ns)
         self.e_range = tuple(float(v) for v in e_range)
         self.net = MLP(4, hidden, n_hidden_layers, 2 * self.ncell, seed=seed, device=device)
@@ -227,6 +286,8 @@ class LightSurrogate:
         L = np.asarray(box.L, np.float32)
         lo = np.array([0, 0, 0, self.e_range[0]], np.float32)
         hi = np.array([L[0], L[1], L[2], self.e_range[1]], np.float32)
+        if self.fold:
+            hi[0] = hi[1] = 0.5 * L[0]
         if self.log_depth_mm > 0:
             lo[2], hi[2] = self._depth(np.float32(0.0)), self._depth(L[2])
         self.net.set_input_norm((lo + hi) / 2, 2.0 / (hi - lo))
@@ -243,6 +304,9 @@ class LightSurrogate:
         """Network inputs (x, y, depth, E) for positions [n, 3] and energies [n]."""
         X = np.column_stack([np.asarray(xyz, np.float32).reshape(-1, 3),
                              np.asarray(e_kev, np.float32).reshape(-1)]).astype(np.float32)
+        if self.fold:
+            xc, yc, _ = fold_xy(X[:, 0], X[:, 1], float(self.box.L[0]))
+            X[:, 0], X[:, 1] = xc, yc
         if self.log_depth_mm > 0:
             X[:, 2] = self._depth(X[:, 2])
         return X
@@ -266,22 +330,26 @@ class LightSurrogate:
         read as relative errors) and standardized mean times (weight c / (c + 20))."""
         p = ts.p
         tm = ts.tmean
+        counts = ts.counts
+        if self.fold:
+            _, _, code = fold_xy(ts.X[:, 0], ts.X[:, 1], float(self.box.L[0]))
+            p, tm, counts = (to_folded(a, code, self.perms) for a in (p, tm, counts))
         T = np.empty((ts.X.shape[0], 2 * self.ncell), np.float32)
         T[:, :self.ncell] = p
         T[:, self.ncell:] = np.nan_to_num((tm - self.t_shift) / self.t_scale, nan=0.0)
         W = np.empty_like(T)
         self.pois_w = float(1.0 / max(p.mean(), 1e-12))
         W[:, :self.ncell] = self.pois_w
-        c = ts.counts.astype(np.float32)
+        c = counts.astype(np.float32)
         W[:, self.ncell:] = 0.25 * c / (c + 20.0)
         return T, W
 
-    def init_output_bias(self, ts: TrainingSet):
-        """Start the log-rate outputs at the log of each cell's mean rate."""
+    def init_output_bias(self, T):
+        """Start the log-rate outputs at the log of each cell's mean rate (T: targets)."""
         par = self.net.get_params()
         n_out = 2 * self.ncell
         b = par.size - n_out
-        par[b:b + self.ncell] = np.log(np.maximum(ts.p.mean(0), 1e-9))
+        par[b:b + self.ncell] = np.log(np.maximum(T[:, :self.ncell].mean(0), 1e-9))
         par[b + self.ncell:] = 0.0
         self.net.set_params(par)
 
@@ -293,7 +361,7 @@ class LightSurrogate:
         import torch
         T, W = self.targets(ts)
         if self.net.get_params()[-2 * self.ncell:].any() == 0:
-            self.init_output_bias(ts)
+            self.init_output_bias(T)
         n = (ts.X.shape[0] // batch) * batch
         if n == 0:
             raise ValueError("training set smaller than one batch")
@@ -325,6 +393,11 @@ class LightSurrogate:
         Y = self.net.forward(self.features(xyz, e_kev))
         p = np.exp(Y[:, :self.ncell])
         t = Y[:, self.ncell:] * self.t_scale + self.t_shift
+        if self.fold:
+            xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+            _, _, code = fold_xy(xyz[:, 0], xyz[:, 1], float(self.box.L[0]))
+            p = from_folded(p, code, self.perms)
+            t = from_folded(t, code, self.perms)
         return p, t
 
     def bake(self, shape=(48, 48, 40), e_kev=None):

@@ -26,10 +26,11 @@ def _sync():
 
 
 def train_for_box(box, op, ro, *, tau_ns, tau_rise_ns=0.0, n_samples=1_000_000,
-                  photons=10_000, epochs=60, batch=8192, seed=11, log_depth_mm=1.0, ctx=None):
+                  photons=10_000, epochs=90, batch=8192, seed=11, log_depth_mm=1.0, fold=True,
+                  ctx=None):
     ctx = ctx or _lib.default_context()
     sur = S.LightSurrogate(box, op, ro, tau_ns=tau_ns, tau_rise_ns=tau_rise_ns, seed=seed,
-                           log_depth_mm=log_depth_mm)
+                           log_depth_mm=log_depth_mm, fold=fold)
     t0 = time.perf_counter()
     ts = sur.generate(n_samples, photons, seed=seed, ctx=ctx)
     _sync()
@@ -39,6 +40,7 @@ def train_for_box(box, op, ro, *, tau_ns, tau_rise_ns=0.0, n_samples=1_000_000,
     t2 = time.perf_counter()
     info = {"n_samples": n_samples, "photons_per_sample": photons, "generate_s": t1 - t0,
             "fit_s": t2 - t1, "epochs": epochs, "batch": batch, "adam_steps": int(losses.size),
+            "symmetry_folded": sur.fold,
             "final_loss": float(losses[-min(50, losses.size):].mean()),
             "train_flops": float(3.0 * sur.net.flops_per_row * (losses.size * batch))}
     info["train_tflops_per_s"] = info["train_flops"] / info["fit_s"] / 1e12

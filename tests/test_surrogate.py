@@ -44,6 +44,48 @@ def test_round_bf16():
     assert abs(r[3] - 3.140625) < 1e-12
 
 
+def _iso_map(x, y, n, pitch, width=4.0):
+    """A light map that depends only on the distance to each SiPM centre: invariant under
+    the square face's mirrors and the x<->y swap.  Cell i + n*j, i along x."""
+    c = (np.arange(n) + 0.5) * pitch
+    dx = x[:, None, None] - c[None, None, :]          # [k, 1, i]
+    dy = y[:, None, None] - c[None, :, None]          # [k, j, 1]
+    return np.exp(-(dx ** 2 + dy ** 2) / width).reshape(len(x), n * n)
+
+
+def test_symmetry_fold_against_closed_form():
+    from crysplight_amd import surrogate as S
+    n, pitch = 8, 3.2
+    L = n * pitch
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(0, L, 4000), rng.uniform(0, L, 4000)
+    xc, yc, code = S.fold_xy(x, y, L)
+    assert np.all((0 <= xc) & (xc <= yc + 1e-12) & (yc <= L / 2 + 1e-12))
+    assert set(np.unique(code)) == set(range(8))
+    perms = S.fold_perms(n)
+    for o in range(8):
+        assert sorted(perms[o]) == list(range(n * n))
+    # the map at the folded point, in folded cell order, unfolds to the map at the point
+    got = S.from_folded(_iso_map(xc, yc, n, pitch), code, perms)
+    assert np.allclose(got, _iso_map(x, y, n, pitch), rtol=1e-12, atol=1e-300)
+    # and folding the actual map gives the folded point's map
+    assert np.allclose(S.to_folded(_iso_map(x, y, n, pitch), code, perms),
+                       _iso_map(xc, yc, n, pitch), rtol=1e-12, atol=1e-300)
+    # a wrong permutation (mirror without the swap) is caught
+    bad = S.from_folded(_iso_map(xc, yc, n, pitch), code & 3, perms)
+    assert not np.allclose(bad, _iso_map(x, y, n, pitch))
+
+
+def test_symmetry_fold_only_when_the_optics_allow_it():
+    from crysplight_amd import surrogate as S
+    from crysplight_amd.params import Box, SipmGrid, readout
+    g = SipmGrid(3.2, 8, 8)
+    assert S.folds(Box((25.6, 25.6, 20.0)), readout(g))
+    assert not S.folds(Box((25.6, 24.0, 20.0)), readout(g))
+    assert not S.folds(Box((30.0, 30.0, 20.0)), readout(g))          # grid short of the face
+    assert not S.folds(Box((25.6, 25.6, 20.0)), readout(g, center=(12.8, 12.8), radius=5.0))
+
+
 gpu = pytest.mark.gpu
 
 
