@@ -76,9 +76,9 @@ __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
 // order (src/philox.jl:34-57).  Generated words wait in a ring of 16 words per lane in
 // LDS: 16 slots x 64 lanes per wavefront, slot-major, so a wavefront's reads and writes
 // are free of bank conflicts whatever slot each lane is at.  A new photon starts with
-// blocks 0 and 1 (rng_start); after that one block is generated per bounce step at a
-// single site for all lanes while the ring has room (rng_prefetch), so a step starts
-// with 8..16 words in hand and a draw is one LDS read.  Only a step that outruns the
+// blocks 0 and 1 (rng_start); after that blocks are generated at the top of each bounce
+// step, for all lanes at once, while the ring has room (rng_prefetch), so a step starts
+// with 9..16 words in hand and a draw is one LDS read.  Only a step that outruns the
 // ring generates a block at the draw itself.
 constexpr uint32_t RING_SLOT = 256u;              // bytes between a lane's ring slots
 constexpr uint32_t RING_POS_MASK = 15u * RING_SLOT;
@@ -138,10 +138,14 @@ __device__ __forceinline__ void rng_start(Rng &r, uint64_t pid, uint32_t k0, uin
     rng_fill(r, k0, k1);
 }
 
-// Top of a step: one block while the ring has four free slots.
+// Top of a step: one block while the ring has four free slots, and a second one for a
+// lane down to eight words.  A step draws four words on average; photons whose steps
+// draw five or six (the escape cone through the air gap keeps its angle at every face of
+// the same axis) would otherwise drain the ring and generate inside the step.
 __device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1)
 {
     if (r.wr - r.rd <= 12u * RING_SLOT) rng_fill(r, k0, k1);
+    if (r.wr - r.rd <= 8u * RING_SLOT) rng_fill(r, k0, k1);
 }
 
 template <bool COUNT>

@@ -80,8 +80,10 @@ __device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep
     return lo;
 }
 
+// Five waves per SIMD: the register target trades a few binary64 constants the compiler
+// would keep resident for one more wave to cover the LDS reads of the Philox rings.
 template <int MODE, bool COUNT, bool MULTIBOX = false>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5, 5)))
 transport_kernel(const KParams P, const KBuffers B)
 {
     // dynamic LDS: the lanes' Philox rings (blockDim.x * 64 B), then in accumulate mode
@@ -376,8 +378,8 @@ struct crysp_ctx {
     size_t lds_per_block = 65536;
     int wg = 256;
     int ppt = 4;
-    int chunk = 256;
-    int refill_min = 16;
+    int chunk = 128;
+    int refill_min = 12;
     bool timing = false;
     double kernel_ms = 0.0;
     int64_t n_launches = 0;
