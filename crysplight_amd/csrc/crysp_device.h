@@ -355,43 +355,72 @@ __device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT
 
 // UNIFIED specular-lobe interaction at the rough crystal-air interface
 // (src/optics.jl:49-81): up to 20 micro-facet tries, fallback ideal specular.
-// Returns true when the photon is transmitted into the gap.
+//
+// facet_try: the first half of a try -- the Box-Muller tilt (two draws), the azimuth
+// (one draw) and the facet normal.  False when the try is void (tilt >= 1.2 rad, or the
+// facet faces away from the photon); the caller then moves on to the next try.
+template <bool COUNT>
+__device__ __forceinline__ bool facet_try(Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                          float ux, float uy, float uz, int a, float sgn,
+                                          float &nfx, float &nfy, float &nfz, float &cosi)
+{
+    oc.add(5);
+    const float u1 = randu<COUNT>(r, P, oc);
+    const float u2 = randu<COUNT>(r, P, oc);
+    const float g = sqrtf(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
+    const float alpha = P.sigma_alpha * fabsf(g);
+    if (alpha >= 1.2f) return false;
+    float sa, ca;
+    f_sincos(alpha, sa, ca);
+    const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    const float b = sa * cp, c = sa * sp;
+    nfx = a == 1 ? ca * sgn : b;
+    nfy = a == 2 ? ca * sgn : (a == 1 ? b : c);
+    nfz = a == 3 ? ca * sgn : c;
+    cosi = ux * nfx + uy * nfy + uz * nfz;
+    return !(cosi <= 0.0f);
+}
+
+// The end of a try that Fresnel-reflected: mirror about the facet, accepted when the
+// result points into the crystal (against the outward normal, whose only nonzero
+// component is on the face axis).
+__device__ __forceinline__ bool facet_reflect(float &ux, float &uy, float &uz, float cosi,
+                                              float nfx, float nfy, float nfz, int a, float sgn)
+{
+    const float rx = ux - 2.0f * cosi * nfx;
+    const float ry = uy - 2.0f * cosi * nfy;
+    const float rz = uz - 2.0f * cosi * nfz;
+    const float rn = a == 1 ? rx * sgn : (a == 2 ? ry * sgn : rz * sgn);
+    if (rn < 0.0f) { ux = rx; uy = ry; uz = rz; return true; }
+    return false;
+}
+
+__device__ __forceinline__ float face_sign(int face)
+{
+    return (face == F_XP || face == F_YP || face == F_BACK) ? 1.0f : -1.0f;
+}
+
+// Tries k0..19 of the interaction.  Returns true when the photon is transmitted into
+// the gap.
 template <bool COUNT>
 __device__ __forceinline__ bool unified_surface(Rng &r, const KParams &P, OpCount<COUNT> &oc,
-                                                float &ux, float &uy, float &uz, int face)
+                                                float &ux, float &uy, float &uz, int face,
+                                                int k0 = 0)
 {
     const int a = face_axis(face);
-    const float sgn = (face == F_XP || face == F_YP || face == F_BACK) ? 1.0f : -1.0f;
-    for (int k = 0; k < 20; ++k) {
-        oc.add(5);
-        const float u1 = randu<COUNT>(r, P, oc);
-        const float u2 = randu<COUNT>(r, P, oc);
-        const float g = sqrtf(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
-        const float alpha = P.sigma_alpha * fabsf(g);
-        if (alpha >= 1.2f) continue;
-        float sa, ca;
-        f_sincos(alpha, sa, ca);
-        const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
-        float sp, cp;
-        f_sincos(phi, sp, cp);
-        const float b = sa * cp, c = sa * sp;
-        const float nfx = a == 1 ? ca * sgn : b;
-        const float nfy = a == 2 ? ca * sgn : (a == 1 ? b : c);
-        const float nfz = a == 3 ? ca * sgn : c;
-        const float cosi = ux * nfx + uy * nfy + uz * nfz;
-        if (cosi <= 0.0f) continue;
+    const float sgn = face_sign(face);
+    for (int k = k0; k < 20; ++k) {
+        float nfx, nfy, nfz, cosi;
+        if (!facet_try<COUNT>(r, P, oc, ux, uy, uz, a, sgn, nfx, nfy, nfz, cosi)) continue;
         float R;
         oc.add(4);
         const bool tir = fresnel(P.q_air, P.n_crystal, 1.0f, cosi, R);
         bool refl = tir;
         if (!tir) refl = randu<COUNT>(r, P, oc) < R;
         if (!refl) return true;
-        const float rx = ux - 2.0f * cosi * nfx;
-        const float ry = uy - 2.0f * cosi * nfy;
-        const float rz = uz - 2.0f * cosi * nfz;
-        // inward check against the outward normal (only its face-axis component is nonzero)
-        const float rn = a == 1 ? rx * sgn : (a == 2 ? ry * sgn : rz * sgn);
-        if (rn < 0.0f) { ux = rx; uy = ry; uz = rz; return false; }
+        if (facet_reflect(ux, uy, uz, cosi, nfx, nfy, nfz, a, sgn)) return false;
     }
     specular(ux, uy, uz, face);
     return false;
@@ -465,37 +494,8 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
     const float cosi = a == 1 ? fabsf(p.ux) : (a == 2 ? fabsf(p.uy) : fabsf(p.uz));
 
     int result = ST_FLYING;
-    if (face == F_BACK || ((P.flags & K_TWO_SIDED) && face == F_FRONT)) {
-        // readout face: crystal -> coupling Fresnel, then PDE (src/kernel.jl:220-251)
-        float R;
-        oc.add(4);
-        const bool tir = fresnel(P.q_coup, P.n_crystal, P.n_coupling, cosi, R);
-        bool refl = tir;
-        if (!tir) refl = randu<COUNT>(r, P, oc) < R;
-        if (refl) {
-            specular(p.ux, p.uy, p.uz, face);
-        } else {
-            const float ex = p.x - P.cx, ey = p.y - P.cy;
-            if (!(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2) {
-                if (randu<COUNT>(r, P, oc) <= P.pde) {
-                    int ii = 1 + (int)(p.x / P.pitch);
-                    int jj = 1 + (int)(p.y / P.pitch);
-                    ii = ii < P.nx ? ii : P.nx;
-                    jj = jj < P.ny ? jj : P.ny;
-                    const int l = (jj - 1) * P.nx + ii;
-                    lin = face == F_FRONT ? -l : l;
-                    oc.add(8);
-                    return ST_DETECTED;
-                }
-                return ST_ABS_SIPM;
-            } else if (randu<COUNT>(r, P, oc) <= P.sur_R) {
-                if (P.flags & K_SUR_SPEC) specular(p.ux, p.uy, p.uz, face);
-                else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
-            } else {
-                return ST_ABS_SUR;
-            }
-        }
-    } else if (!(P.flags & K_AIR_GAP)) {
+    const bool readout_face = face == F_BACK || ((P.flags & K_TWO_SIDED) && face == F_FRONT);
+    if (!readout_face && !(P.flags & K_AIR_GAP)) {
         // front-painted wrap in optical contact (src/kernel.jl:252-262)
         if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
             if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
@@ -504,25 +504,68 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
             return ST_ABS_WALL;
         }
     } else {
-        // back-painted: crystal -> air, then the reflector (src/kernel.jl:263-293)
-        bool transmitted;
-        if (P.sigma_alpha > 0.0f) {
-            transmitted = unified_surface<COUNT>(r, P, oc, p.ux, p.uy, p.uz, face);
-        } else {
+        // Readout face: crystal -> coupling Fresnel against the face normal, then PDE
+        // (src/kernel.jl:220-251).  Back-painted wrap: crystal -> air Fresnel, against a
+        // sampled micro-facet normal when the surface is rough, then the reflector
+        // (src/kernel.jl:263-293).  Both Fresnel evaluations (and the first UNIFIED try's)
+        // run as one, with per-lane indices and cosine; each lane's draws keep their order.
+        const float sgn = face_sign(face);
+        const bool rough = !readout_face && P.sigma_alpha > 0.0f;
+        float cf = cosi, nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
+        bool ok = true;                              // the (first) try reaches Fresnel
+        if (rough) ok = facet_try<COUNT>(r, P, oc, p.ux, p.uy, p.uz, a, sgn, nfx, nfy, nfz, cf);
+        bool refl = false;
+        if (ok) {
             float R;
             oc.add(4);
-            const bool tir = fresnel(P.q_air, P.n_crystal, 1.0f, cosi, R);
-            bool refl = tir;
+            const bool tir = fresnel(readout_face ? P.q_coup : P.q_air, P.n_crystal,
+                                     readout_face ? P.n_coupling : 1.0f, cf, R);
+            refl = tir;
             if (!tir) refl = randu<COUNT>(r, P, oc) < R;
-            transmitted = !refl;
-            if (refl) specular(p.ux, p.uy, p.uz, face);
         }
-        if (transmitted) {
-            if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
-                if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
-                else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+        if (readout_face) {
+            if (refl) {
+                specular(p.ux, p.uy, p.uz, face);
             } else {
-                return ST_ABS_WALL;
+                const float ex = p.x - P.cx, ey = p.y - P.cy;
+                if (!(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2) {
+                    if (randu<COUNT>(r, P, oc) <= P.pde) {
+                        int ii = 1 + (int)(p.x / P.pitch);
+                        int jj = 1 + (int)(p.y / P.pitch);
+                        ii = ii < P.nx ? ii : P.nx;
+                        jj = jj < P.ny ? jj : P.ny;
+                        const int l = (jj - 1) * P.nx + ii;
+                        lin = face == F_FRONT ? -l : l;
+                        oc.add(8);
+                        return ST_DETECTED;
+                    }
+                    return ST_ABS_SIPM;
+                } else if (randu<COUNT>(r, P, oc) <= P.sur_R) {
+                    if (P.flags & K_SUR_SPEC) specular(p.ux, p.uy, p.uz, face);
+                    else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+                } else {
+                    return ST_ABS_SUR;
+                }
+            }
+        } else {
+            bool transmitted;
+            if (!rough) {
+                transmitted = !refl;
+                if (refl) specular(p.ux, p.uy, p.uz, face);
+            } else if (ok && !refl) {
+                transmitted = true;
+            } else if (ok && facet_reflect(p.ux, p.uy, p.uz, cf, nfx, nfy, nfz, a, sgn)) {
+                transmitted = false;
+            } else {
+                transmitted = unified_surface<COUNT>(r, P, oc, p.ux, p.uy, p.uz, face, 1);
+            }
+            if (transmitted) {
+                if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
+                    if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
+                    else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
+                } else {
+                    return ST_ABS_WALL;
+                }
             }
         }
     }
