@@ -76,9 +76,9 @@ __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
 // order (src/philox.jl:34-57).  Generated words wait in a ring of 16 words per lane in
 // LDS: 16 slots x 64 lanes per wavefront, slot-major, so a wavefront's reads and writes
 // are free of bank conflicts whatever slot each lane is at.  A new photon starts with
-// blocks 0 and 1 (rng_start); after that blocks are generated at the top of each bounce
-// step, for all lanes at once, while the ring has room (rng_prefetch), so a step starts
-// with 9..16 words in hand and a draw is one LDS read.  Only a step that outruns the
+// block 0 (rng_start); after that blocks are generated at the top of each bounce step,
+// for all lanes at once, while the ring has room (rng_prefetch), so a step starts with
+// 9..16 words in hand and a draw is one LDS read.  Only a step that outruns the
 // ring generates a block at the draw itself.
 constexpr uint32_t RING_SLOT = 256u;              // bytes between a lane's ring slots
 constexpr uint32_t RING_POS_MASK = 15u * RING_SLOT;
@@ -130,11 +130,11 @@ __device__ __forceinline__ void rng_fill(Rng &r, uint32_t k0, uint32_t k1)
     r.wr += 4u * RING_SLOT;
 }
 
-// A new photon: blocks 0 and 1.
+// A new photon: block 0, enough for the emission (at most four draws); the step-top
+// sites fill the ring further together with the other lanes.
 __device__ __forceinline__ void rng_start(Rng &r, uint64_t pid, uint32_t k0, uint32_t k1)
 {
     rng_init(r, pid);
-    rng_fill(r, k0, k1);
     rng_fill(r, k0, k1);
 }
 
@@ -495,14 +495,16 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
 
     int result = ST_FLYING;
     const bool readout_face = face == F_BACK || ((P.flags & K_TWO_SIDED) && face == F_FRONT);
+    // After the optics of the face, at most one more draw decides the photon: the PDE
+    // (readout face, inside the active area), the surround reflectivity (readout face,
+    // outside it) or the wrap reflectivity (a wrapped face the photon reached).  The three
+    // are one draw site with a per-lane threshold.
+    int kind = 0;                                   // 1 PDE, 2 surround, 3 wrap
+    float thr = 0.0f;
     if (!readout_face && !(P.flags & K_AIR_GAP)) {
         // front-painted wrap in optical contact (src/kernel.jl:252-262)
-        if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
-            if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
-            else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
-        } else {
-            return ST_ABS_WALL;
-        }
+        kind = 3;
+        thr = P.wrap_R;
     } else {
         // Readout face: crystal -> coupling Fresnel against the face normal, then PDE
         // (src/kernel.jl:220-251).  Back-painted wrap: crystal -> air Fresnel, against a
@@ -528,24 +530,9 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
                 specular(p.ux, p.uy, p.uz, face);
             } else {
                 const float ex = p.x - P.cx, ey = p.y - P.cy;
-                if (!(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2) {
-                    if (randu<COUNT>(r, P, oc) <= P.pde) {
-                        int ii = 1 + (int)(p.x / P.pitch);
-                        int jj = 1 + (int)(p.y / P.pitch);
-                        ii = ii < P.nx ? ii : P.nx;
-                        jj = jj < P.ny ? jj : P.ny;
-                        const int l = (jj - 1) * P.nx + ii;
-                        lin = face == F_FRONT ? -l : l;
-                        oc.add(8);
-                        return ST_DETECTED;
-                    }
-                    return ST_ABS_SIPM;
-                } else if (randu<COUNT>(r, P, oc) <= P.sur_R) {
-                    if (P.flags & K_SUR_SPEC) specular(p.ux, p.uy, p.uz, face);
-                    else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
-                } else {
-                    return ST_ABS_SUR;
-                }
+                const bool inside = !(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2;
+                kind = inside ? 1 : 2;
+                thr = inside ? P.pde : P.sur_R;
             }
         } else {
             bool transmitted;
@@ -560,14 +547,28 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
                 transmitted = unified_surface<COUNT>(r, P, oc, p.ux, p.uy, p.uz, face, 1);
             }
             if (transmitted) {
-                if (randu<COUNT>(r, P, oc) <= P.wrap_R) {
-                    if (P.flags & K_WRAP_SPEC) specular(p.ux, p.uy, p.uz, face);
-                    else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
-                } else {
-                    return ST_ABS_WALL;
-                }
+                kind = 3;
+                thr = P.wrap_R;
             }
         }
+    }
+    if (kind != 0) {
+        const bool pass = randu<COUNT>(r, P, oc) <= thr;
+        if (kind == 1) {
+            if (!pass) return ST_ABS_SIPM;
+            int ii = 1 + (int)(p.x / P.pitch);
+            int jj = 1 + (int)(p.y / P.pitch);
+            ii = ii < P.nx ? ii : P.nx;
+            jj = jj < P.ny ? jj : P.ny;
+            const int l = (jj - 1) * P.nx + ii;
+            lin = face == F_FRONT ? -l : l;
+            oc.add(8);
+            return ST_DETECTED;
+        }
+        if (!pass) return kind == 2 ? ST_ABS_SUR : ST_ABS_WALL;
+        const bool spec = kind == 2 ? (P.flags & K_SUR_SPEC) != 0u : (P.flags & K_WRAP_SPEC) != 0u;
+        if (spec) specular(p.ux, p.uy, p.uz, face);
+        else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
     }
     ++p.bounces;
     return p.bounces >= P.maxb ? ST_CAP : result;
