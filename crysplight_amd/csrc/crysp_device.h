@@ -255,6 +255,32 @@ __device__ __forceinline__ void f_sincos(float xf, float &sf, float &cf)
     cf = (float)cv;
 }
 
+// sin and cos of a micro-facet tilt alpha >= 0.  Below 1/8 rad, where alpha = sigma |g|
+// almost always lies, the Taylor series go straight from alpha with no reduction: the
+// first omitted terms, a^13/13! and a^14/14!, are below 2^-66 relative.  Larger tilts
+// take f_sincos.
+__device__ __forceinline__ void f_sincos_tilt(float af, float &sf, float &cf)
+{
+    if (!(af < 0.125f)) { f_sincos(af, sf, cf); return; }
+    const double a = (double)af;
+    const double z = a * a;
+    double ps = -2.505210838544172e-08;                      // -1/11!
+    ps = __builtin_fma(ps, z, 2.7557319223985893e-06);       // 1/9!
+    ps = __builtin_fma(ps, z, -0.0001984126984126984);       // -1/7!
+    ps = __builtin_fma(ps, z, 0.008333333333333333);         // 1/5!
+    ps = __builtin_fma(ps, z, -0.16666666666666666);         // -1/3!
+    const double sn = __builtin_fma(a * z, ps, a);
+    double pc = 2.08767569878681e-09;                        // 1/12!
+    pc = __builtin_fma(pc, z, -2.755731922398589e-07);       // -1/10!
+    pc = __builtin_fma(pc, z, 2.48015873015873e-05);         // 1/8!
+    pc = __builtin_fma(pc, z, -0.001388888888888889);        // -1/6!
+    pc = __builtin_fma(pc, z, 0.041666666666666664);         // 1/4!
+    pc = __builtin_fma(pc, z, -0.5);                         // -1/2!
+    const double cs = __builtin_fma(z, pc, 1.0);
+    sf = (float)sn;
+    cf = (float)cs;
+}
+
 __device__ __forceinline__ float f_cos(float x)
 {
     float s, c;
@@ -371,7 +397,7 @@ __device__ __forceinline__ bool facet_try(Rng &r, const KParams &P, OpCount<COUN
     const float alpha = P.sigma_alpha * fabsf(g);
     if (alpha >= 1.2f) return false;
     float sa, ca;
-    f_sincos(alpha, sa, ca);
+    f_sincos_tilt(alpha, sa, ca);
     const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
     float sp, cp;
     f_sincos(phi, sp, cp);

@@ -340,6 +340,7 @@ __global__ void debug_math_kernel(int which, const float *x, int64_t n, float *y
         if (which == 0) y[i] = f_log(v);
         else if (which == 3) y[i] = sqrtf(v);
         else if (which == 4) y[i] = f_cos(v);
+        else if (which == 5 || which == 6) { f_sincos_tilt(v, s, c); y[i] = which == 5 ? s : c; }
         else { f_sincos(v, s, c); y[i] = which == 1 ? s : c; }
     }
 }
@@ -1106,7 +1107,7 @@ int crysp_debug_draws(crysp_ctx *c, uint64_t seed, uint64_t pid0, int64_t n_pids
 
 int crysp_debug_math(crysp_ctx *c, int32_t which, const float *x, int64_t n, float *y)
 {
-    if (!c || !x || !y || n < 0 || which < 0 || which > 4) return fail(CRYSP_EINVAL, "bad arguments");
+    if (!c || !x || !y || n < 0 || which < 0 || which > 6) return fail(CRYSP_EINVAL, "bad arguments");
     if (n == 0) return CRYSP_OK;
     HIPCHK(hipSetDevice(c->device));
     const float *dx;

@@ -54,6 +54,20 @@ def test_sincos_on_tilt_domain(ctx, fn):
     assert np.count_nonzero(g != oracle.correctly_rounded(fn, x)) == 0
 
 
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_sincos_tilt_path(ctx, fn):
+    """The facet-tilt sin/cos: every 7th binary32 value in [0, 1.2) (series below 1/8,
+    the general routine above), and every value in [1/16, 1/8], where the series'
+    truncation is largest, plus the switch point."""
+    lo, hi = np.float32(0.0625).view(np.uint32), np.float32(0.125).view(np.uint32)
+    x = np.concatenate([oracle.tilt_domain(7),
+                        np.arange(lo, hi + 2, dtype=np.uint32).view(np.float32)])
+    g = T.debug_math(fn + "_tilt", x, ctx=ctx)
+    want = oracle.correctly_rounded(fn, x)
+    bad = np.flatnonzero(g != want)
+    assert bad.size == 0, (x[bad[:10]], g[bad[:10]], want[bad[:10]])
+
+
 def test_sqrt_correctly_rounded(ctx):
     """sqrt is evaluated on [0, 4): every 3rd float there, plus all of [0.25, 1)."""
     hi = np.float32(4.0).view(np.uint32)
