@@ -60,11 +60,12 @@ __device__ __forceinline__ uint4 philox10(uint32_t c0, uint32_t c1, uint32_t c2,
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         // one 32x32->64 product each (v_mad_u64_u32) gives both halves
+        // and one three-input XOR (v_bitop3_b32, truth table 0x96) for each new word
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        c0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
         c1 = (uint32_t)p1;
-        c2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
@@ -116,12 +117,18 @@ struct OpCount {
     }
 };
 
+// Byte address of ring position pos: its slot bits over the lane's base (disjoint bits).
+__device__ __forceinline__ uint32_t ring_addr(const Rng &r, uint32_t pos)
+{
+    return (pos & RING_POS_MASK) | r.base;
+}
+
 // Generate the next block into the ring (the caller guarantees four free slots).  The
 // writer's slot is a multiple of four, so the block's slots are consecutive.
 __device__ __forceinline__ void rng_fill(Rng &r, uint32_t k0, uint32_t k1)
 {
     const uint4 b = philox10(r.lo, r.hi, r.blk, 0u, k0, k1);
-    char *q = r.ring + ((r.wr & RING_POS_MASK) | r.base);
+    char *q = r.ring + ring_addr(r, r.wr);
     *(uint32_t *)(q) = b.x;
     *(uint32_t *)(q + RING_SLOT) = b.y;
     *(uint32_t *)(q + 2 * RING_SLOT) = b.z;
@@ -154,7 +161,7 @@ __device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<C
     if (r.rd == r.wr) rng_fill(r, P.key0, P.key1);
     if (COUNT && ((r.rd / RING_SLOT) & 3u) == 0u) oc.add(2);   // a block's first word
     oc.add(1);
-    const uint32_t x = *(const uint32_t *)(r.ring + ((r.rd & RING_POS_MASK) | r.base));
+    const uint32_t x = *(const uint32_t *)(r.ring + ring_addr(r, r.rd));
     r.rd += RING_SLOT;
     return x;
 }
