@@ -295,6 +295,22 @@ __device__ __forceinline__ float f_cos(float x)
     return c;
 }
 
+// Correctly rounded binary32 sqrt for 0 and normal arguments -- every argument in the
+// transport is 0 or at least 2^-25, so the input scaling of the general sequence is left
+// out: the hardware estimate (within 1 ulp) moves to a neighbour when the fma residual
+// of that neighbour shows it is the better one.
+__device__ __forceinline__ float f_sqrt(float x)
+{
+    const float s0 = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s0) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s0) + 1u);
+    const float rd = __builtin_fmaf(-sd, s0, x);
+    const float ru = __builtin_fmaf(-su, s0, x);
+    float s = rd <= 0.0f ? sd : s0;
+    s = ru > 0.0f ? su : s;
+    return s;
+}
+
 // Julia Base semantics used by the reference
 __device__ __forceinline__ float jmax0(float v) { return (v != v) ? v : (v > 0.0f ? v : 0.0f); }
 __device__ __forceinline__ float jclamp(float x, float lo, float hi)
@@ -309,7 +325,7 @@ __device__ __forceinline__ bool fresnel(float q, float n1, float n2, float cosi,
 {
     const float sin2t = (q * q) * (1.0f - cosi * cosi);
     if (sin2t >= 1.0f) { R = 1.0f; return true; }
-    const float cost = sqrtf(1.0f - sin2t);
+    const float cost = f_sqrt(1.0f - sin2t);
     const float a = (n1 * cosi - n2 * cost) / (n1 * cosi + n2 * cost);
     const float b = (n1 * cost - n2 * cosi) / (n1 * cost + n2 * cosi);
     R = 0.5f * (a * a + b * b);
@@ -335,8 +351,8 @@ __device__ __forceinline__ void lambertian(Rng &r, const KParams &P, OpCount<COU
                                            int face, float &ux, float &uy, float &uz)
 {
     oc.add(6);
-    const float ct = sqrtf(randu<COUNT>(r, P, oc));
-    const float st = sqrtf(jmax0(1.0f - ct * ct));
+    const float ct = f_sqrt(randu<COUNT>(r, P, oc));
+    const float st = f_sqrt(jmax0(1.0f - ct * ct));
     const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
     float sp, cp;
     f_sincos(phi, sp, cp);
@@ -363,17 +379,17 @@ __device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT
         const float u2 = randu<COUNT>(r, P, oc);
         if (2.0f * u2 <= 1.0f + mu * mu) break;
     }
-    const float st = sqrtf(jmax0(1.0f - mu * mu));
+    const float st = f_sqrt(jmax0(1.0f - mu * mu));
     const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
     float sp, cp;
     f_sincos(phi, sp, cp);
     const float cb = st * cp, sb = st * sp;
     float t1x, t1y, t1z, inv;
     if (fabsf(uz) < 0.9f) {
-        inv = 1.0f / sqrtf(ux * ux + uy * uy);
+        inv = 1.0f / f_sqrt(ux * ux + uy * uy);
         t1x = -uy * inv; t1y = ux * inv; t1z = 0.0f;
     } else {
-        inv = 1.0f / sqrtf(uy * uy + uz * uz);
+        inv = 1.0f / f_sqrt(uy * uy + uz * uz);
         t1x = 0.0f; t1y = -uz * inv; t1z = uy * inv;
     }
     const float t2x = uy * t1z - uz * t1y;
@@ -382,7 +398,7 @@ __device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT
     const float vx = mu * ux + cb * t1x + sb * t2x;
     const float vy = mu * uy + cb * t1y + sb * t2y;
     const float vz = mu * uz + cb * t1z + sb * t2z;
-    inv = 1.0f / sqrtf(vx * vx + vy * vy + vz * vz);
+    inv = 1.0f / f_sqrt(vx * vx + vy * vy + vz * vz);
     ux = vx * inv; uy = vy * inv; uz = vz * inv;
 }
 
@@ -400,7 +416,7 @@ __device__ __forceinline__ bool facet_try(Rng &r, const KParams &P, OpCount<COUN
     oc.add(5);
     const float u1 = randu<COUNT>(r, P, oc);
     const float u2 = randu<COUNT>(r, P, oc);
-    const float g = sqrtf(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
+    const float g = f_sqrt(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
     const float alpha = P.sigma_alpha * fabsf(g);
     if (alpha >= 1.2f) return false;
     float sa, ca;
@@ -466,7 +482,7 @@ __device__ __forceinline__ void emit(Rng &r, const KParams &P, OpCount<COUNT> &o
 {
     const float u1 = randu<COUNT>(r, P, oc);
     const float z = 1.0f - 2.0f * u1;
-    const float rr = sqrtf(jmax0(1.0f - z * z));
+    const float rr = f_sqrt(jmax0(1.0f - z * z));
     const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
     float sp, cp;
     f_sincos(phi, sp, cp);

@@ -338,7 +338,7 @@ __global__ void debug_math_kernel(int which, const float *x, int64_t n, float *y
         const float v = x[i];
         float s, c;
         if (which == 0) y[i] = f_log(v);
-        else if (which == 3) y[i] = sqrtf(v);
+        else if (which == 3) y[i] = f_sqrt(v);
         else if (which == 4) y[i] = f_cos(v);
         else if (which == 5 || which == 6) { f_sincos_tilt(v, s, c); y[i] = which == 5 ? s : c; }
         else { f_sincos(v, s, c); y[i] = which == 1 ? s : c; }

@@ -69,11 +69,14 @@ def test_sincos_tilt_path(ctx, fn):
 
 
 def test_sqrt_correctly_rounded(ctx):
-    """sqrt is evaluated on [0, 4): every 3rd float there, plus all of [0.25, 1)."""
-    hi = np.float32(4.0).view(np.uint32)
-    x = np.concatenate([np.arange(0, hi, 3, dtype=np.uint32).view(np.float32),
-                        np.arange(np.float32(0.25).view(np.uint32),
-                                  np.float32(1.0).view(np.uint32), dtype=np.uint32).view(np.float32)])
+    """sqrt is evaluated at 0 and on [2^-25, 34) (the Box-Muller radius reaches
+    sqrt(-2 log 2^-24) = sqrt(33.3)): every float of [0.25, 4) (two binades: the
+    correction does not depend on the exponent), every 5th float from 2^-25 to 34, zero."""
+    as_f = lambda a: a.astype(np.uint32).view(np.float32)
+    b = lambda v: int(np.float32(v).view(np.uint32))
+    x = np.concatenate([as_f(np.arange(b(0.25), b(4.0))),
+                        as_f(np.arange(b(2.0 ** -25), b(34.0), 5)),
+                        np.zeros(1, np.float32)])
     g = T.debug_math("sqrt", x, ctx=ctx)
     assert np.array_equal(g, np.sqrt(x))
 
