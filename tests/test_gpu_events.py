@@ -126,3 +126,60 @@ def test_events_bad_event_index_is_an_error(ctx, bgo_events):
                      dep.e_kev[:5], 4)
     with pytest.raises(Exception):
         E.transport_events(K.C2_BOX, K.C2_OP, RO8, bad, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+
+
+def test_events_empty_tables_and_zero_photon_deposits(ctx):
+    """No deposits at all; and deposits with zero photons mixed with events that have no
+    deposit (src/events.jl:59-91 keeps such events with empty maps)."""
+    empty = E.Deposits(np.zeros((0, 4), np.float32), np.zeros(0, np.int32),
+                       np.zeros(0, np.int32), np.zeros(0, np.float32), 5)
+    r = E.transport_events(K.C2_BOX, K.C2_OP, RO8, empty, seed=3, tau_ns=K.C2_TAU, ctx=ctx)
+    assert r.n_photons == 0 and not r.maps.any() and not r.npe.any()
+    assert np.all(np.isinf(r.tmin))
+    rng = np.random.default_rng(4)
+    nd = 40
+    xyzt = np.column_stack([rng.uniform(1, 24, nd), rng.uniform(1, 24, nd), rng.uniform(1, 19, nd),
+                            rng.uniform(0, 5, nd)]).astype(np.float32)
+    nph = rng.integers(0, 400, nd).astype(np.int32)
+    nph[::3] = 0
+    ev = np.sort(rng.choice(np.arange(0, 30, 2), nd)).astype(np.int32)   # odd events empty
+    dep = E.Deposits(xyzt, nph, ev, np.zeros(nd, np.float32), 31)
+    res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=3, tau_ns=K.C2_TAU, ctx=ctx)
+    assert res.n_photons == int(nph.sum())
+    _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 3, K.C2_TAU), 31)
+    assert not res.npe[1::2].any()
+
+
+def test_events_multibox_matches_per_deposit_oracle(ctx):
+    """Deposits of three crystal thicknesses in one queue: photon g of the table keeps
+    stream pid0 + g + 1 and is transported in its deposit's crystal."""
+    boxes = [Box((25.6, 25.6, t)) for t in (10.0, 20.0, 30.0)]
+    rng = np.random.default_rng(8)
+    nd = 45
+    dep_box = rng.integers(0, 3, nd).astype(np.int32)
+    L = np.array([b.L for b in boxes], np.float32)[dep_box]
+    xyz = rng.uniform(0.05, 0.95, (nd, 3)).astype(np.float32) * L
+    xyzt = np.column_stack([xyz, rng.uniform(0, 3, nd)]).astype(np.float32)
+    nph = rng.integers(0, 300, nd).astype(np.int32)
+    ev = np.sort(rng.integers(0, 12, nd)).astype(np.int32)
+    dep = E.Deposits(xyzt, nph, ev, np.zeros(nd, np.float32), 12)
+    pid0 = 1000
+    res = E.transport_events_multibox(boxes, dep_box, K.C2_OP, RO8, dep, seed=9, tau_ns=K.C2_TAU,
+                                      pid0=pid0, ctx=ctx)
+    maps = np.zeros((12, 64), np.int64)
+    tmin = np.full((12, 64), np.inf, np.float32)
+    g0 = 0
+    for d in range(nd):
+        n = int(nph[d])
+        if n:
+            rec = oracle.transport_records(boxes[dep_box[d]], K.C2_OP, RO8, n_photons=n, seed=9,
+                                           pos=tuple(xyzt[d, :3]), t0s=[xyzt[d, 3]],
+                                           pid_offset=pid0 + g0, tau_ns=K.C2_TAU)
+            det = rec["status"] == 1
+            cell = np.abs(rec["idx"][det].astype(np.int64)) - 1
+            np.add.at(maps[ev[d]], cell, 1)
+            np.minimum.at(tmin[ev[d]], cell, rec["t"][det])
+        g0 += n
+    assert np.array_equal(res.maps.reshape(12, -1), maps)
+    assert np.array_equal(res.npe, maps.sum(1))
+    assert np.array_equal(res.tmin.reshape(12, -1), tmin)

@@ -140,3 +140,18 @@ def test_op_counts_match_oracle(ctx):
         ctx.set_op_counting(False)
     r = oracle.transport_records(K.C1_BOX, K.C1_OP, K.C1_RO, count_ops=True, **kw)
     assert list(ops) == [r["ops"][n] for n in oracle.OPCOUNT_NAMES]
+
+
+def test_zero_photons_and_bounce_cap(ctx):
+    """An empty call writes nothing and leaves the accumulator empty; a small max_bounces
+    caps photons (CAP status, src/transport.jl:228-232) exactly where the oracle does."""
+    rec = T.transport_records(K.C1_BOX, K.C1_OP, K.C1_RO, n_photons=0, seed=1, pos=K.C1_POS,
+                              tau_ns=K.C1_TAU, ctx=ctx)
+    assert rec.status.size == 0
+    acc = T.transport_accumulate(K.C1_BOX, K.C1_OP, K.C1_RO, K.C1_TB, n_photons=0, seed=1,
+                                 pos=K.C1_POS, tau_ns=K.C1_TAU, ctx=ctx)
+    assert acc.total_terminated() == 0 and not acc.counts.any() and np.all(np.isinf(acc.first_ns))
+    g, r, frac = _records_case(ctx, K.C1_BOX, K.C1_OP, K.C1_RO, 20_000, 77, K.C1_POS,
+                               K.C1_TAU, maxb=3)
+    assert frac == 1.0
+    assert np.count_nonzero(g.status == 5) > 1000
