@@ -27,7 +27,8 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int BM = 64, BN = 64, BK = 32;            // BK: granularity of K (and K slices)
+constexpr int KP = 128;                             // K staged in LDS per phase
 constexpr int LDS_PAD = 8;                          // bf16 elements of row padding
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
@@ -55,6 +56,8 @@ struct GemmArgs {
     const __bf16 *A; int lda; int a_t;   // a_t: A stored [K][M] (element (m,k) at k*lda+m)
     const __bf16 *B; int ldb; int b_t;   // b_t: B stored [N][K] (element (k,n) at n*ldb+k)
     float *C; int ldc;                   // fp32 output (EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU: Z)
+    int64_t c_slice;                     // EPI_F32 split over K (gridDim.z > 1): the partial
+                                         // product of K slice z goes to C + z * c_slice
     __bf16 *H; int ldh;                  // bf16 output
     const float *bias;                   // [N]
     const float *Zin;                    // EPI_GELU_BWD: pre-activation, ldc layout
@@ -63,8 +66,8 @@ struct GemmArgs {
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
 {
-    __shared__ __bf16 sA[BM][BK + LDS_PAD];        // [m][k]
-    __shared__ __bf16 sB[BN][BK + LDS_PAD];        // [n][k]
+    __shared__ __bf16 sA[BM][KP + LDS_PAD];        // [m][k]
+    __shared__ __bf16 sB[BN][KP + LDS_PAD];        // [n][k]
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
@@ -73,34 +76,46 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
 
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        // stage the A tile (64 x 32) and B tile (32 x 64): 8 elements per thread each
-        {
-            const int e = tid * 8;
-            if (!g.a_t) {
-                const int r = e / BK, c = e % BK;    // row m, 8 consecutive k
-                const bf16x8 v = *(const bf16x8 *)(g.A + (size_t)(m0 + r) * g.lda + k0 + c);
-                *(bf16x8 *)&sA[r][c] = v;
-            } else {
-                const int kk = e / BM, mm = e % BM;  // row k, 8 consecutive m
-                const bf16x8 v = *(const bf16x8 *)(g.A + (size_t)(k0 + kk) * g.lda + m0 + mm);
+    // K slice of this workgroup (gridDim.z slices of K / gridDim.z, a multiple of BK),
+    // staged KP at a time: every load of a phase is in flight at once, then each wave
+    // runs KP/16 MFMA steps on it
+    const int kc = g.K / (int)gridDim.z;
+    const int kbeg = (int)blockIdx.z * kc, kend = kbeg + kc;
+    for (int k0 = kbeg; k0 < kend; k0 += KP) {
+        const int kp = min(KP, kend - k0);         // a multiple of BK
+        const int chunks = kp * (BM / 8);          // 8-element chunks per tile (BM == BN)
+        constexpr int CPT = KP * (BM / 8) / 256;   // chunks per thread in a full phase
+        bf16x8 va[CPT], vb[CPT];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) sA[mm + j][kk] = v[j];
+        for (int u = 0; u < CPT; ++u) {            // all loads first ...
+            const int e = (tid + u * 256) * 8;
+            if (e >= chunks * 8) break;
+            va[u] = !g.a_t ? *(const bf16x8 *)(g.A + (size_t)(m0 + e / kp) * g.lda + k0 + e % kp)
+                           : *(const bf16x8 *)(g.A + (size_t)(k0 + e / BM) * g.lda + m0 + e % BM);
+            vb[u] = g.b_t ? *(const bf16x8 *)(g.B + (size_t)(n0 + e / kp) * g.ldb + k0 + e % kp)
+                          : *(const bf16x8 *)(g.B + (size_t)(k0 + e / BN) * g.ldb + n0 + e % BN);
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {            // ... then the LDS stores
+            const int e = (tid + u * 256) * 8;
+            if (e >= chunks * 8) break;
+            if (!g.a_t) {
+                *(bf16x8 *)&sA[e / kp][e % kp] = va[u];           // row m, 8 consecutive k
+            } else {
+                const int kk = e / BM, mm = e % BM;               // row k, 8 consecutive m
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sA[mm + j][kk] = va[u][j];
             }
             if (g.b_t) {
-                const int r = e / BK, c = e % BK;    // row n, 8 consecutive k
-                const bf16x8 v = *(const bf16x8 *)(g.B + (size_t)(n0 + r) * g.ldb + k0 + c);
-                *(bf16x8 *)&sB[r][c] = v;
+                *(bf16x8 *)&sB[e / kp][e % kp] = vb[u];           // row n, 8 consecutive k
             } else {
-                const int kk = e / BN, nn = e % BN;  // row k, 8 consecutive n
-                const bf16x8 v = *(const bf16x8 *)(g.B + (size_t)(k0 + kk) * g.ldb + n0 + nn);
+                const int kk = e / BN, nn = e % BN;               // row k, 8 consecutive n
 #pragma unroll
-                for (int j = 0; j < 8; ++j) sB[nn + j][kk] = v[j];
+                for (int j = 0; j < 8; ++j) sB[nn + j][kk] = vb[u][j];
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
+        for (int kk = 0; kk < kp; kk += 16) {
             // lane l: row/col l % 32, k = kk + 8 * (l / 32) .. +7
             const int r = lane & 31, kq = kk + 8 * (lane >> 5);
             const bf16x8 a = *(const bf16x8 *)&sA[wm + r][kq];
@@ -117,7 +132,7 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
         const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
         float v = acc[i];
         if (EPI == EPI_F32) {
-            g.C[(size_t)row * g.ldc + col] = v;
+            g.C[(size_t)blockIdx.z * g.c_slice + (size_t)row * g.ldc + col] = v;
         } else if (EPI == EPI_BIAS_F32) {
             g.C[(size_t)row * g.ldc + col] = v + g.bias[col];
         } else if (EPI == EPI_BIAS_GELU) {
@@ -134,9 +149,12 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
 
 // ---------------------------------------------------------------- small kernels ----
 // inputs fp32 [M x n_in] -> bf16 [Mpad x kin_pad], normalized: (x - shift) * scale
+// bidx (optional, device): the batch index, X advanced by *bidx * bstride (graph replays)
 __global__ void pack_inputs(const float *X, int M, int Mpad, int n_in, int kin_pad,
-                            const float *shift, const float *scale, __bf16 *out)
+                            const float *shift, const float *scale, __bf16 *out,
+                            const int64_t *bidx, int64_t bstride)
 {
+    if (bidx) X += *bidx * bstride;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)Mpad * kin_pad) return;
     const int r = (int)(i / kin_pad), c = (int)(i % kin_pad);
@@ -149,59 +167,122 @@ __global__ void pack_inputs(const float *X, int M, int Mpad, int n_in, int kin_p
 //   columns c <  n_pois: Poisson negative log-likelihood of a log-rate y for an observed
 //                        rate t: w (e^y - t y), gradient w (e^y - t);
 //   columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t).
-__global__ void loss_grad(const float *Y, const float *T, const float *W, int M, int Mpad, int N,
-                          int n_pois, float inv_wsum, __bf16 *dY, float *loss_acc)
+__global__ void __launch_bounds__(256) loss_grad(const float *Y, const float *T, const float *W,
+                                                 int M, int Mpad, int N, int n_pois,
+                                                 float inv_wsum, __bf16 *dY, float *loss_part,
+                                                 const int64_t *bidx, int64_t bstride)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ float red[4];
+    if (bidx) {                                     // targets and weights of batch *bidx
+        T += *bidx * bstride;
+        if (W) W += *bidx * bstride;
+    }
     float l = 0.0f;
-    if (i < (int64_t)Mpad * N) {
+    const int64_t tot = (int64_t)Mpad * N;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(i / N), c = (int)(i % N);
         float g = 0.0f;
         if (r < M) {
             const float w = W ? W[i] : 1.0f;
             if (c < n_pois) {
                 const float e = __expf(Y[i]);
-                l = w * (e - T[i] * Y[i]) * inv_wsum;
+                l += w * (e - T[i] * Y[i]) * inv_wsum;
                 g = w * (e - T[i]) * inv_wsum;
             } else {
                 const float d = Y[i] - T[i];
-                l = w * d * d * inv_wsum;
+                l += w * d * d * inv_wsum;
                 g = 2.0f * w * d * inv_wsum;
             }
         }
         dY[i] = to_bf16(g);
     }
-    // wave reduction, one atomic per wave
+    // fixed-order reduction: the loss is the same on every run
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) l += __shfl_xor(l, m);
-    if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss_acc, l);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// column sums of a bf16 matrix [M x N] (bias gradient) into fp32 out (zeroed before):
-// block (x: 256 columns, y: a slice of 256 rows)
-__global__ void col_sum(const __bf16 *G, int M, int N, float *out)
+// Partial column sums of a bf16 matrix [M x N] (bias gradient): part[s][c] = sum of rows
+// s*rows_per_slice .. of column c.  Block x: 256 columns, block y: slice s.
+__global__ void col_sum_part(const __bf16 *G, int M, int N, int rows_per_slice, float *part)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const int r0 = blockIdx.y * 256, r1 = min(M, r0 + 256);
+    const int r0 = blockIdx.y * rows_per_slice, r1 = min(M, r0 + rows_per_slice);
     float s = 0.0f;
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s += from_bf16(G[(size_t)r * N + c]);
-    atomicAdd(&out[c], s);
+    part[(size_t)blockIdx.y * N + c] = s;
+}
+
+// out[r][c] = sum over s = 0..S-1, in order, of P[s * slice + r * ldp + c], for c < cols
+// (split-K partial products, bias partials); out row stride ldo.
+__global__ void sum_slices(const float *P, int S, int64_t slice, int rows, int cols, int ldp,
+                           float *out, int ldo)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)rows * cols;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / cols), c = (int)(i % cols);
+        const float *q = P + (size_t)r * ldp + c;
+        float s = 0.0f;
+        int k = 0;
+        for (; k + 8 <= S; k += 8) {              // eight loads in flight, added in order
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = q[(size_t)(k + j) * slice];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        for (; k < S; ++k) s += q[(size_t)k * slice];
+        out[(size_t)r * ldo + c] = s;
+    }
+}
+
+// Sum of n values by one workgroup in a fixed order, into out[*idx] (out[0] without idx).
+__global__ void __launch_bounds__(256) sum_small(const float *x, int n, float *out,
+                                                 const int64_t *idx)
+{
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[idx ? *idx : 0] = red[0];
 }
 
 // Adam on fp32 master weights; refreshes the bf16 copies in both layouts
 // Wt [out][in_pad] (forward, k-contiguous) and Wn [in][out] (backward data gradient).
+// dstep (optional, device): the steps taken so far; the bias corrections c1, c2 then
+// follow from it on the device (graph replays)
 __global__ void adam_update(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
-                            float b2, float eps, float c1, float c2, float wd)
+                            float b2, float eps, float c1, float c2, float wd, const int64_t *dstep)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (dstep) {
+        const float t = (float)(*dstep + 1);
+        c1 = 1.0f / (1.0f - powf(b1, t));
+        c2 = 1.0f / (1.0f - powf(b2, t));
+    }
     const float gi = g[i] + wd * w[i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
     w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// end of a replayed training step: next batch, one more Adam step
+__global__ void train_step_end(int64_t *ctr)
+{
+    if (threadIdx.x == 0) { ctr[0] += 1; ctr[1] += 1; }
 }
 
 __global__ void refresh_bf16(const float *w, int out_dim, int in_dim, int in_pad, __bf16 *Wt,
@@ -353,10 +434,30 @@ struct crysp_mlp {
     __bf16 *dY = nullptr;
     std::vector<__bf16 *> dZ;                       // [cap][out] per layer
     __bf16 *dH = nullptr;                           // scratch [cap][hidden]
-    float *dW0 = nullptr;                           // layer-0 weight gradient at padded width
+    float *part = nullptr;                          // split-K partial weight gradients
+    size_t part_cap = 0;                            // floats
+    float *bpart = nullptr;                         // bias-gradient slice partials
+    float *lpart = nullptr;                         // loss partials [LOSS_BLOCKS]
     float *loss = nullptr;
+    int64_t *ctr = nullptr;                         // device: batch index, Adam steps (graph replays)
+    int64_t hctr[2] = {0, 0};
     double flops_fwd_per_row = 0.0;
 };
+
+// K slices for a weight-gradient product (output M x N, reduction over the batch K): the
+// output alone is only a few 64x64 tiles, so K is split until about 512 workgroups run,
+// keeping each slice a multiple of BK and at least two BK steps.
+constexpr int SPLIT_TARGET_WG = 512;
+static int weight_grad_slices(int M, int N, int K)
+{
+    const int tiles = (M / BM) * (N / BN);
+    int ks = 1;
+    while (tiles * ks * 2 <= SPLIT_TARGET_WG && K % (ks * 2 * BK) == 0 && K / (ks * 2) >= 2 * BK)
+        ks *= 2;
+    return ks;
+}
+constexpr int LOSS_BLOCKS = 256;                    // loss partials per step
+constexpr int BIAS_ROWS = 32;                       // rows per bias-gradient slice
 
 static int mlp_alloc_batch(crysp_mlp *n, int cap)
 {
@@ -373,27 +474,44 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
     SCHK(hipMalloc(&n->Y, (size_t)cap * n->n_out * 4));
     SCHK(hipMalloc(&n->dY, (size_t)cap * n->n_out * 2));
     SCHK(hipMalloc(&n->dH, (size_t)cap * std::max(n->hidden, n->n_out) * 2));
+    size_t part = 0;
+    int bias_cols = 0;
     for (int l = 0; l < n->n_layers; ++l) {
         SCHK(hipMalloc(&n->Z[l], (size_t)cap * n->dout[l] * 4));
         SCHK(hipMalloc(&n->H[l], (size_t)cap * n->dout[l] * 2));
         SCHK(hipMalloc(&n->dZ[l], (size_t)cap * n->dout[l] * 2));
+        // the most slices any batch can get: tiles * slices stays within SPLIT_TARGET_WG
+        const int tiles = (n->dout[l] / BM) * (n->dpad[l] / BN);
+        int ks_max = 1;
+        while (tiles * ks_max * 2 <= SPLIT_TARGET_WG) ks_max *= 2;
+        part = std::max(part, (size_t)ks_max * n->dout[l] * n->dpad[l]);
+        bias_cols = std::max(bias_cols, n->dout[l]);
     }
+    fre(n->part); fre(n->bpart); fre(n->lpart);
+    n->part = nullptr; n->bpart = nullptr; n->lpart = nullptr;
+    SCHK(hipMalloc(&n->part, part * 4));
+    n->part_cap = part;
+    SCHK(hipMalloc(&n->bpart, (size_t)((cap + BIAS_ROWS - 1) / BIAS_ROWS) * bias_cols * 4));
+    SCHK(hipMalloc(&n->lpart, LOSS_BLOCKS * 4));
     n->cap = cap;
     return CRYSP_OK;
 }
 
 template <int EPI>
-static void gemm(hipStream_t s, const GemmArgs &a)
+static void gemm(hipStream_t s, const GemmArgs &a, int k_slices = 1)
 {
-    dim3 grid(a.N / BN, a.M / BM);
+    dim3 grid(a.N / BN, a.M / BM, k_slices);
     hipLaunchKernelGGL(gemm_bf16<EPI>, grid, dim3(256), 0, s, a);
 }
 
-static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad)
+
+static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad,
+                           const int64_t *bidx = nullptr)
 {
     const int64_t tot = (int64_t)Mpad * n->kin_pad;
     hipLaunchKernelGGL(pack_inputs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n->stream,
-                       Xdev, M, Mpad, n->n_in, n->kin_pad, n->shift, n->scale, n->X);
+                       Xdev, M, Mpad, n->n_in, n->kin_pad, n->shift, n->scale, n->X, bidx,
+                       (int64_t)M * n->n_in);
     const __bf16 *in = n->X;
     int ld_in = n->kin_pad;
     for (int l = 0; l < n->n_layers; ++l) {
@@ -415,37 +533,46 @@ static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad)
     return CRYSP_OK;
 }
 
-// gradients of the weighted MSE into n->g (fp32 flat), loss into n->loss
-static int mlp_backward_dev(crysp_mlp *n, const float *T, const float *W, int M, int Mpad, double wsum)
+// gradients of the weighted loss into n->g (fp32 flat), the loss into *loss_out (device;
+// n->loss when null)
+static int mlp_backward_dev(crysp_mlp *n, const float *T, const float *W, int M, int Mpad, double wsum,
+                            float *loss_out = nullptr, const int64_t *bidx = nullptr)
 {
     const int64_t tot = (int64_t)Mpad * n->n_out;
-    SCHK(hipMemsetAsync(n->loss, 0, 4, n->stream));
-    hipLaunchKernelGGL(loss_grad, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n->stream,
-                       n->Y, T, W, M, Mpad, n->n_out, n->n_pois, (float)(1.0 / wsum), n->dY, n->loss);
+    const int lblocks = (int)std::min<int64_t>(LOSS_BLOCKS, (tot + 255) / 256);
+    hipLaunchKernelGGL(loss_grad, dim3(lblocks), dim3(256), 0, n->stream, n->Y, T, W, M, Mpad,
+                       n->n_out, n->n_pois, (float)(1.0 / wsum), n->dY, n->lpart, bidx,
+                       (int64_t)M * n->n_out);
+    hipLaunchKernelGGL(sum_small, dim3(1), dim3(256), 0, n->stream, n->lpart, lblocks,
+                       loss_out ? loss_out : n->loss, loss_out ? bidx : nullptr);
     const __bf16 *dcur = n->dY;                    // gradient w.r.t. layer l's output (pre-act)
     for (int l = n->n_layers - 1; l >= 0; --l) {
         const __bf16 *hin = l == 0 ? n->X : n->H[l - 1];
         const int kin = n->dpad[l], nout = n->dout[l];
-        // dW[out][in] = dZ^T[out x M] * Hin[M x in]   (A = dZ stored [M][out] -> a_t)
+        // dW[out][in] = dZ^T[out x M] * Hin[M x in]   (A = dZ stored [M][out] -> a_t), as
+        // split-K partial products summed in a fixed order; at layer 0 only the real
+        // input columns of the padded width are kept
         {
+            const int ks = weight_grad_slices(nout, kin, Mpad);
+            if ((size_t)ks * nout * kin > n->part_cap) return sfail(CRYSP_EINVAL, "split-K buffer too small");
             GemmArgs a{};
             a.M = nout; a.N = kin; a.K = Mpad;
             a.A = dcur; a.lda = nout; a.a_t = 1;
             a.B = hin; a.ldb = kin; a.b_t = 0;
-            if (kin == n->din[l]) {
-                a.C = n->g + n->woff[l]; a.ldc = kin;
-                gemm<EPI_F32>(n->stream, a);
-            } else {
-                // padded input width: full-width gradient, then keep the real columns
-                a.C = n->dW0; a.ldc = kin;
-                gemm<EPI_F32>(n->stream, a);
-                SCHK(hipMemcpy2DAsync(n->g + n->woff[l], (size_t)n->din[l] * 4, n->dW0, (size_t)kin * 4,
-                                      (size_t)n->din[l] * 4, nout, hipMemcpyDeviceToDevice, n->stream));
-            }
+            a.C = n->part; a.ldc = kin; a.c_slice = (int64_t)nout * kin;
+            gemm<EPI_F32>(n->stream, a, ks);
+            const int64_t cells = (int64_t)nout * n->din[l];
+            hipLaunchKernelGGL(sum_slices, dim3((unsigned)std::min<int64_t>(1024, (cells + 255) / 256)),
+                               dim3(256), 0, n->stream, n->part, ks, a.c_slice, nout, n->din[l], kin,
+                               n->g + n->woff[l], n->din[l]);
         }
-        SCHK(hipMemsetAsync(n->g + n->boff[l], 0, (size_t)nout * 4, n->stream));
-        hipLaunchKernelGGL(col_sum, dim3((nout + 255) / 256, (Mpad + 255) / 256), dim3(256), 0,
-                           n->stream, dcur, Mpad, nout, n->g + n->boff[l]);
+        {
+            const int slices = (Mpad + BIAS_ROWS - 1) / BIAS_ROWS;
+            hipLaunchKernelGGL(col_sum_part, dim3((nout + 255) / 256, slices), dim3(256), 0,
+                               n->stream, dcur, Mpad, nout, BIAS_ROWS, n->bpart);
+            hipLaunchKernelGGL(sum_slices, dim3((nout + 255) / 256), dim3(256), 0, n->stream,
+                               n->bpart, slices, (int64_t)nout, 1, nout, nout, n->g + n->boff[l], nout);
+        }
         if (l == 0) break;
         // dZ_{l-1} = (dZ_l [M x out] * W_l [out x in]) * gelu'(Z_{l-1});  B = Wn [in][out] -> b_t
         {
@@ -529,7 +656,7 @@ crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_
     ok = ok && hipMalloc(&n->v, n->n_params * 4) == hipSuccess;
     ok = ok && hipMalloc(&n->shift, 32 * 4) == hipSuccess && hipMalloc(&n->scale, 32 * 4) == hipSuccess;
     ok = ok && hipMalloc(&n->loss, 64) == hipSuccess;
-    ok = ok && hipMalloc(&n->dW0, (size_t)hidden * n->kin_pad * 4) == hipSuccess;
+    ok = ok && hipMalloc(&n->ctr, 2 * sizeof(int64_t)) == hipSuccess;
     for (int l = 0; l < n->n_layers && ok; ++l) {
         __bf16 *a = nullptr, *b = nullptr;
         ok = hipMalloc(&a, (size_t)n->dout[l] * n->dpad[l] * 2) == hipSuccess;
@@ -583,8 +710,8 @@ void crysp_mlp_destroy(crysp_mlp *n)
     (void)hipSetDevice(n->device);
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     auto fre = [](void *q) { if (q) (void)hipFree(q); };
-    fre(n->p); fre(n->g); fre(n->m); fre(n->v); fre(n->shift); fre(n->scale); fre(n->loss);
-    fre(n->dW0);
+    fre(n->p); fre(n->g); fre(n->m); fre(n->v); fre(n->shift); fre(n->scale); fre(n->loss); fre(n->ctr);
+    fre(n->part); fre(n->bpart); fre(n->lpart);
     for (auto q : n->Wt) fre(q);
     for (auto q : n->Wn) fre(q);
     fre(n->X); fre(n->Y); fre(n->dY); fre(n->dH);
@@ -699,20 +826,58 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
     const double wsum = (double)batch * n->n_out * (W ? mean_row_weight : 1.0f);
     float *dl = nullptr;
     if (losses && n_batches > 0) SCHK(hipMalloc(&dl, (size_t)n_batches * 4));
-    for (int64_t b = 0; b < n_batches; ++b) {
-        const float *x = X + (size_t)b * batch * n->n_in;
-        const float *t = T + (size_t)b * batch * n->n_out;
-        const float *w = W ? W + (size_t)b * batch * n->n_out : nullptr;
-        if ((rc = mlp_forward_dev(n, x, batch, batch))) return rc;
-        if ((rc = mlp_backward_dev(n, t, w, batch, batch, wsum))) return rc;
-        n->step += 1;
-        const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)n->step));
-        const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)n->step));
-        hipLaunchKernelGGL(adam_update, dim3((unsigned)((n->n_params + 255) / 256)), dim3(256), 0,
-                           n->stream, n->p, n->g, n->m, n->v, (int)n->n_params, lr, beta1, beta2,
-                           eps, c1, c2, weight_decay);
+    // One step (forward, backward, Adam, bf16 refresh) is captured once and replayed per
+    // batch: the batch index and the Adam step count live on the device and advance at
+    // the end of each replay, so the ~30 small launches of a step cost one graph launch.
+    const unsigned adam_blocks = (unsigned)((n->n_params + 255) / 256);
+    auto enqueue_step = [&](const float *x, const float *t, const float *w, float *lout,
+                            const int64_t *bidx, const int64_t *dstep, float c1, float c2) -> int {
+        int r;
+        if ((r = mlp_forward_dev(n, x, batch, batch, bidx))) return r;
+        if ((r = mlp_backward_dev(n, t, w, batch, batch, wsum, lout, bidx))) return r;
+        hipLaunchKernelGGL(adam_update, dim3(adam_blocks), dim3(256), 0, n->stream, n->p, n->g,
+                           n->m, n->v, (int)n->n_params, lr, beta1, beta2, eps, c1, c2,
+                           weight_decay, dstep);
         mlp_refresh(n);
-        if (dl) SCHK(hipMemcpyAsync(dl + b, n->loss, 4, hipMemcpyDeviceToDevice, n->stream));
+        return CRYSP_OK;
+    };
+    hipGraphExec_t exec = nullptr;
+    if (n_batches > 1) {
+        n->hctr[0] = 0;
+        n->hctr[1] = n->step;
+        SCHK(hipMemcpyAsync(n->ctr, n->hctr, sizeof n->hctr, hipMemcpyHostToDevice, n->stream));
+        SCHK(hipStreamSynchronize(n->stream));
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            rc = enqueue_step(X, T, W, dl, n->ctr, n->ctr + 1, 0.0f, 0.0f);
+            if (rc == CRYSP_OK)
+                hipLaunchKernelGGL(train_step_end, dim3(1), dim3(64), 0, n->stream, n->ctr);
+            const hipError_t e = hipStreamEndCapture(n->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); if (dl) (void)hipFree(dl); return rc; }
+            if (e != hipSuccess || !g || hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+                exec = nullptr;
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (!exec) (void)hipGetLastError();         // capture unavailable: launch eagerly
+    }
+    if (exec) {
+        for (int64_t b = 0; b < n_batches; ++b) SCHK(hipGraphLaunch(exec, n->stream));
+        n->step += (int)n_batches;
+        SCHK(hipStreamSynchronize(n->stream));
+        (void)hipGraphExecDestroy(exec);
+    } else {
+        for (int64_t b = 0; b < n_batches; ++b) {
+            const float *x = X + (size_t)b * batch * n->n_in;
+            const float *t = T + (size_t)b * batch * n->n_out;
+            const float *w = W ? W + (size_t)b * batch * n->n_out : nullptr;
+            n->step += 1;
+            const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)n->step));
+            const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)n->step));
+            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, nullptr, c1, c2))) {
+                if (dl) (void)hipFree(dl);
+                return rc;
+            }
+        }
     }
     if (dl) {
         SCHK(hipMemcpyAsync(losses, dl, (size_t)n_batches * 4, hipMemcpyDeviceToHost, n->stream));
