@@ -112,6 +112,19 @@ def test_gamma_cross_sections(csi, bgo):
     assert csi.sigma_macro(10.0)[2] > 0.0
 
 
+@pytest.mark.parametrize("name,L_mm", [("csi", 37.2), ("bgo", 22.4)])
+def test_gamma_interaction_probability_closed_form(name, L_mm, request):
+    """Gammas entering a slab along its normal interact (deposit energy) with probability
+    1 - exp(-mu L), mu = (incoherent + photoelectric) at 511 keV: coherent scattering is
+    not simulated (src/gamma/Gamma.jl:7).  Within 4 binomial sigma over 40k events."""
+    mat = request.getfixturevalue(name)
+    inc, pe, pair = mat.sigma_macro(0.511)
+    p = 1.0 - math.exp(-(inc + pe + pair) * L_mm / 10.0)
+    dep, truth = E.gamma_events(P.Box((48.0, 48.0, L_mm)), mat, 1000.0, n_events=40_000, seed=21)
+    got = float((truth["edep"] > 0).mean())
+    assert abs(got - p) < 4 * math.sqrt(p * (1 - p) / 40_000), (got, p)
+
+
 def test_gamma_loglog_interpolation_hits_table(csi):
     """At a grid energy the interpolation returns the tabulated mu/rho * rho; between
     two rows it is the straight line in log-log (src/gamma/nist_data.jl:59-71)."""

@@ -76,3 +76,39 @@ def test_crysp_baselines_reproduce_published_collection(ctx):
             if want is not None and as_published(g) != Decimal(want):
                 bad.append((run, key, f"{g:.3f}", want))
     assert not bad, bad
+
+
+# The gamma pipeline's first-physics table (latex/crysplight_e2e.tex:110-128, 175): 5000
+# events per crystal from runs/gammas_{csitl,bgo}.toml.  The reference draws its gamma
+# histories from Julia's Xoshiro, so the events differ and the agreement is statistical:
+# within 4 combined standard errors for the photopeak mean and the two fractions, 5 for
+# the photopeak width.  (interacted, photopeak fraction, photopeak pe, FWHM/mean or None)
+GAMMA = {
+    "gammas_csitl_crysp": (0.790, 0.509, 9798.0, 0.0240),
+    "gammas_bgo_crysp_pde045": (0.874, 0.753, 2832.0, 0.0443),
+    "gammas_bgo_crysp": (None, None, 2518.0, None),
+}
+
+
+def test_gamma_pipeline_reproduces_first_physics(ctx):
+    import math
+    from crysplight_amd.config import run_gamma_campaign
+    bad = []
+    for run, (inter, peak, pe, fwhm) in GAMMA.items():
+        r = run_gamma_campaign(HERE / "runs" / f"{run}.toml", ctx=ctx, write=False, verbose=True)
+        n = r["n_events"]
+        n_peak = r["photopeak_fraction"] * n
+        sd = r["photopeak_fwhm_over_mean"] / 2.355 * r["photopeak_pe_mean"]
+        n_pub = (peak if peak is not None else r["photopeak_fraction"]) * 5000
+        checks = [("photopeak_pe_mean", pe, 4 * sd * math.sqrt(1 / n_peak + 1 / n_pub))]
+        if inter is not None:
+            checks.append(("interacted_fraction", inter, 4 * math.sqrt(2 * inter * (1 - inter) / n)))
+            checks.append(("photopeak_fraction", peak, 4 * math.sqrt(2 * peak * (1 - peak) / n)))
+        if fwhm is not None:
+            checks.append(("photopeak_fwhm_over_mean", fwhm, 5 * fwhm * math.sqrt(1 / (2 * n_peak) + 1 / (2 * n_pub))))
+        for key, want, tol in checks:
+            got = r[key]
+            print(f"{run} {key}: {got:.5g} vs published {want:.5g} (tolerance {tol:.3g})")
+            if abs(got - want) > tol:
+                bad.append((run, key, got, want, tol))
+    assert not bad, bad
