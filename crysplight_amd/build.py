@@ -80,7 +80,7 @@ def build(verbose: bool = False) -> Path:
 
 def build_oracle(verbose: bool = False) -> None:
     """Build the CPU oracle (test infrastructure) under oracle/."""
-    r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
+    r = subprocess.run(["make", "-B", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
     if verbose and r.stdout.strip():

@@ -4,19 +4,20 @@ write_output       <- write_output (src/output.jl:4-52): counts, first_time_ns, 
 symmetry_residual  <- symmetry_residual (src/output.jl:58-62)
 write_events       <- the events.h5 writer of scripts/shoot_gammas.jl:12-17,60-90
 
-No HDF5 library is available in this environment, so the datasets are written to a
-NumPy .npz archive under the reference's names and dtypes, with the arrays in the
-order h5py presents the reference's Julia (column-major) files: light counts as
-(nbins+1, ny, nx), first_time_ns (ny, nx); events maps/tmin (n, ny, nx) with maps
-narrowed to UInt16 as in the reference, xyz (n, 3).  Attributes go to "attrs" as JSON.
+Both files are HDF5 (written by crysplight_amd.h5) under the reference's names and
+dtypes, with the arrays in the order h5py presents the reference's Julia (column-major)
+files: light counts as (nbins+1, ny, nx), first_time_ns (ny, nx); events maps/tmin
+(n, ny, nx) with maps narrowed to UInt16 as in the reference, xyz (n, 3).  Root
+attributes carry the reference's keys.
 """
 from __future__ import annotations
 
-import json
 import shutil
 from pathlib import Path
 
 import numpy as np
+
+from . import h5
 
 
 def symmetry_residual(m: np.ndarray) -> float:
@@ -35,18 +36,16 @@ def write_output(st, acc, elapsed: float, write: bool = True, verbose: bool = Tr
         st.outdir.mkdir(parents=True, exist_ok=True)
         shutil.copy(st.cfg_path, st.outdir / st.cfg_path.name)
         surf = st.cfg.get("surface", {})
-        attrs = {"tag": st.tag, "n_photons": n, "seed": st.seed, "bin_ns": st.tb.bin_ns,
-                 "nbins": st.tb.nbins, "crystal": st.cfg["crystal"]["material"],
-                 "wrap": st.cfg["wrap"]["material"], "pde": st.op.pde, "ndet": acc.ndet,
+        attrs = {"tag": st.tag, "n_photons": n, "seed": st.seed, "bin_ns": np.float32(st.tb.bin_ns),
+                 "nbins": np.int32(st.tb.nbins), "crystal": st.cfg["crystal"]["material"],
+                 "wrap": st.cfg["wrap"]["material"], "pde": np.float32(st.op.pde), "ndet": acc.ndet,
                  "nabs_bulk": acc.nabs_bulk, "nabs_wall": acc.nabs_wall,
                  "nabs_sipm": acc.nabs_sipm, "ncap": acc.ncap, "nabs_surround": acc.nabs_sur,
                  "nscat": acc.nscat, "finish": surf.get("finish", "backpainted"),
-                 "sigma_alpha_deg": float(surf.get("sigma_alpha_deg", 0.0)),
-                 "backend": "hip-gfx950"}
-        np.savez(st.outdir / "light.npz",
-                 counts=np.ascontiguousarray(acc.counts.transpose(2, 1, 0)),
-                 first_time_ns=np.ascontiguousarray(acc.first_ns.T),
-                 attrs=np.array(json.dumps(attrs)))
+                 "sigma_alpha_deg": float(surf.get("sigma_alpha_deg", 0.0))}
+        h5.write(st.outdir / "light.h5",
+                 {"counts": np.asarray(acc.counts, np.uint32).transpose(2, 1, 0),
+                  "first_time_ns": np.asarray(acc.first_ns, np.float32).T}, attrs)
     m = acc.counts.sum(axis=2)
     sym = symmetry_residual(m)
     mean_b = acc.sum_bounces_det / acc.ndet if acc.ndet else float("nan")
@@ -70,17 +69,16 @@ def write_events(c, r: dict, elapsed: float, outdir, write: bool = True, verbose
         outdir.mkdir(parents=True, exist_ok=True)
         shutil.copy(c.cfg_path, outdir / c.cfg_path.name)
         attrs = {"tag": c.tag, "n_events": nev, "seed": c.seed,
-                 "crystal": c.cfg["crystal"]["material"], "pde": c.op.pde, "backend": "hip-gfx950",
+                 "crystal": c.cfg["crystal"]["material"], "pde": np.float32(c.op.pde), "backend": "hip-gfx950",
                  "int_type_key": "-1 = no interaction, 0 = direct photoelectric, X>=1 = X Compton scatters",
                  "tmin_nohit": "Inf"}
-        np.savez(outdir / "events.npz",
-                 edep_kev=np.asarray(r["edep"], np.float32), npe=np.asarray(r["npe"], np.int32),
-                 maps=np.minimum(maps, 65535).astype(np.uint16),
-                 tmin_ns=np.asarray(r["tmin"], np.float32),
-                 xyz1_mm=np.asarray(r["xyz1"], np.float32), e1_kev=np.asarray(r["e1"], np.float32),
-                 xyz2_mm=np.asarray(r["xyz2"], np.float32), e2_kev=np.asarray(r["e2"], np.float32),
-                 er_kev=np.asarray(r["er"], np.float32), int_type=np.asarray(r["int_type"], np.int8),
-                 n_int=np.asarray(r["n_int"], np.int16), attrs=np.array(json.dumps(attrs)))
+        h5.write(outdir / "events.h5", {
+            "edep_kev": np.asarray(r["edep"], np.float32), "npe": np.asarray(r["npe"], np.int32),
+            "maps": np.minimum(maps, 65535).astype(np.uint16), "tmin_ns": np.asarray(r["tmin"], np.float32),
+            "xyz1_mm": np.asarray(r["xyz1"], np.float32), "e1_kev": np.asarray(r["e1"], np.float32),
+            "xyz2_mm": np.asarray(r["xyz2"], np.float32), "e2_kev": np.asarray(r["e2"], np.float32),
+            "er_kev": np.asarray(r["er"], np.float32), "int_type": np.asarray(r["int_type"], np.int8),
+            "n_int": np.asarray(r["n_int"], np.int16)}, attrs)
     edep = np.asarray(r["edep"])
     npe = np.asarray(r["npe"])
     inter = edep > 0

@@ -1,6 +1,6 @@
 """Gamma-event driver: python -m crysplight_amd.shoot_gammas <campaign.toml> [more...]
 (scripts/shoot_gammas.jl): 511 keV gammas -> deposits -> optical transport on the MI355X
--> output/<tag>/events.npz."""
+-> output/<tag>/events.h5."""
 import sys
 
 from .config import run_gamma_campaign
