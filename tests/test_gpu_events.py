@@ -5,6 +5,7 @@ import pytest
 
 import oracle
 from crysplight_amd import events as E
+from crysplight_amd._lib import CryspError
 from crysplight_amd.params import Box, SipmGrid, readout
 from tests import cases as K
 
@@ -118,6 +119,24 @@ def test_events_schedule_independent_and_autotune(ctx, bgo_events):
         assert ctx.get_schedule() == best
     finally:
         ctx.set_schedule(*old)
+
+
+def test_events_queue_parameters_do_not_change_results(ctx, bgo_events):
+    """The work-queue grab size and the lane-refill batch only reorder which lane runs which
+    photon: every (chunk, refill_min) gives the oracle's maps, including a chunk of one
+    photon, chunks smaller and larger than a wave, and refills of one lane or of 64."""
+    dep, _ = bgo_events
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    try:
+        for chunk, rmin in [(1, 1), (5, 3), (64, 64), (4096, 12), (1 << 20, 7)]:
+            ctx.set_queue(chunk, rmin)
+            res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+            _check(res, ref, dep.n_events)
+        for bad in [(0, 12), (1 << 21, 12), (128, 0), (128, 65)]:
+            with pytest.raises(CryspError):
+                ctx.set_queue(*bad)
+    finally:
+        ctx.set_queue(128, 12)
 
 
 def test_events_bad_event_index_is_an_error(ctx, bgo_events):
