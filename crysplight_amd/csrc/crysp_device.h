@@ -190,6 +190,17 @@ __device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> 
 
 // log(x) = e ln2 + 2 atanh(s),  x = 2^e m,  m in [sqrt(2)/2, sqrt(2)),  s = (m-1)/(m+1),
 // |s| <= 0.1716: the atanh series to s^23 leaves < 2^-55 relative.
+// a * b + c for a binary64 constant c, which the VOP3 fma reads from a scalar register
+// pair.  Written as __builtin_fma, the compiler selects the two-address fmac form, whose
+// addend must be the destination VGPR pair: every Horner step then copied its constant into
+// VGPRs (one v_mov_b64 per term) and the resident constants held 20-30 VGPRs.
+__device__ __forceinline__ double fma_k(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
 __device__ __forceinline__ float f_log(float x)
 {
     const uint32_t b = __float_as_uint(x);
@@ -209,16 +220,16 @@ __device__ __forceinline__ float f_log(float x)
     const double s = num * rc;
     const double z = s * s;
     double p = 0.043478260869565216;                         // 1/23
-    p = __builtin_fma(p, z, 0.047619047619047616);           // 1/21
-    p = __builtin_fma(p, z, 0.05263157894736842);            // 1/19
-    p = __builtin_fma(p, z, 0.058823529411764705);           // 1/17
-    p = __builtin_fma(p, z, 0.06666666666666667);            // 1/15
-    p = __builtin_fma(p, z, 0.07692307692307693);            // 1/13
-    p = __builtin_fma(p, z, 0.09090909090909091);            // 1/11
-    p = __builtin_fma(p, z, 0.1111111111111111);             // 1/9
-    p = __builtin_fma(p, z, 0.14285714285714285);            // 1/7
-    p = __builtin_fma(p, z, 0.2);                            // 1/5
-    p = __builtin_fma(p, z, 0.3333333333333333);             // 1/3
+    p = fma_k(p, z, 0.047619047619047616);           // 1/21
+    p = fma_k(p, z, 0.05263157894736842);            // 1/19
+    p = fma_k(p, z, 0.058823529411764705);           // 1/17
+    p = fma_k(p, z, 0.06666666666666667);            // 1/15
+    p = fma_k(p, z, 0.07692307692307693);            // 1/13
+    p = fma_k(p, z, 0.09090909090909091);            // 1/11
+    p = fma_k(p, z, 0.1111111111111111);             // 1/9
+    p = fma_k(p, z, 0.14285714285714285);            // 1/7
+    p = fma_k(p, z, 0.2);                            // 1/5
+    p = fma_k(p, z, 0.3333333333333333);             // 1/3
     const double lm = 2.0 * __builtin_fma(s * z, p, s);      // 2 atanh(s)
     const double ed = (double)e;
     // ln2 in two parts: ed * LN2_HI is exact for |e| < 2^21
@@ -236,21 +247,21 @@ __device__ __forceinline__ void f_sincos(float xf, float &sf, float &cf)
     r = __builtin_fma(-kd, 2.0222662487959506e-21, r);
     const double z = r * r;
     double ps = 2.8114572543455206e-15;                      // 1/17!
-    ps = __builtin_fma(ps, z, -7.647163731819816e-13);
-    ps = __builtin_fma(ps, z, 1.6059043836821613e-10);
-    ps = __builtin_fma(ps, z, -2.505210838544172e-08);
-    ps = __builtin_fma(ps, z, 2.7557319223985893e-06);
-    ps = __builtin_fma(ps, z, -0.0001984126984126984);
-    ps = __builtin_fma(ps, z, 0.008333333333333333);
-    ps = __builtin_fma(ps, z, -0.16666666666666666);
+    ps = fma_k(ps, z, -7.647163731819816e-13);
+    ps = fma_k(ps, z, 1.6059043836821613e-10);
+    ps = fma_k(ps, z, -2.505210838544172e-08);
+    ps = fma_k(ps, z, 2.7557319223985893e-06);
+    ps = fma_k(ps, z, -0.0001984126984126984);
+    ps = fma_k(ps, z, 0.008333333333333333);
+    ps = fma_k(ps, z, -0.16666666666666666);
     const double sn = __builtin_fma(r * z, ps, r);           // sin r
     double pc = 4.779477332387385e-14;                       // 1/16!
-    pc = __builtin_fma(pc, z, -1.1470745597729725e-11);
-    pc = __builtin_fma(pc, z, 2.08767569878681e-09);
-    pc = __builtin_fma(pc, z, -2.755731922398589e-07);
-    pc = __builtin_fma(pc, z, 2.48015873015873e-05);
-    pc = __builtin_fma(pc, z, -0.001388888888888889);
-    pc = __builtin_fma(pc, z, 0.041666666666666664);
+    pc = fma_k(pc, z, -1.1470745597729725e-11);
+    pc = fma_k(pc, z, 2.08767569878681e-09);
+    pc = fma_k(pc, z, -2.755731922398589e-07);
+    pc = fma_k(pc, z, 2.48015873015873e-05);
+    pc = fma_k(pc, z, -0.001388888888888889);
+    pc = fma_k(pc, z, 0.041666666666666664);
     pc = __builtin_fma(pc, z, -0.5);
     const double cs = __builtin_fma(z, pc, 1.0);             // cos r
     const int q = (int)kd & 3;
@@ -272,16 +283,16 @@ __device__ __forceinline__ void f_sincos_tilt(float af, float &sf, float &cf)
     const double a = (double)af;
     const double z = a * a;
     double ps = -2.505210838544172e-08;                      // -1/11!
-    ps = __builtin_fma(ps, z, 2.7557319223985893e-06);       // 1/9!
-    ps = __builtin_fma(ps, z, -0.0001984126984126984);       // -1/7!
-    ps = __builtin_fma(ps, z, 0.008333333333333333);         // 1/5!
-    ps = __builtin_fma(ps, z, -0.16666666666666666);         // -1/3!
+    ps = fma_k(ps, z, 2.7557319223985893e-06);       // 1/9!
+    ps = fma_k(ps, z, -0.0001984126984126984);       // -1/7!
+    ps = fma_k(ps, z, 0.008333333333333333);         // 1/5!
+    ps = fma_k(ps, z, -0.16666666666666666);         // -1/3!
     const double sn = __builtin_fma(a * z, ps, a);
     double pc = 2.08767569878681e-09;                        // 1/12!
-    pc = __builtin_fma(pc, z, -2.755731922398589e-07);       // -1/10!
-    pc = __builtin_fma(pc, z, 2.48015873015873e-05);         // 1/8!
-    pc = __builtin_fma(pc, z, -0.001388888888888889);        // -1/6!
-    pc = __builtin_fma(pc, z, 0.041666666666666664);         // 1/4!
+    pc = fma_k(pc, z, -2.755731922398589e-07);       // -1/10!
+    pc = fma_k(pc, z, 2.48015873015873e-05);         // 1/8!
+    pc = fma_k(pc, z, -0.001388888888888889);        // -1/6!
+    pc = fma_k(pc, z, 0.041666666666666664);         // 1/4!
     pc = __builtin_fma(pc, z, -0.5);                         // -1/2!
     const double cs = __builtin_fma(z, pc, 1.0);
     sf = (float)sn;

@@ -80,10 +80,10 @@ __device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep
     return lo;
 }
 
-// Five waves per SIMD: the register target trades a few binary64 constants the compiler
-// would keep resident for one more wave to cover the LDS reads of the Philox rings.
+// Seven waves per SIMD (at most 72 VGPRs): the waves cover the LDS reads of the Philox
+// rings and each other's dependent binary64 chains.
 template <int MODE, bool COUNT, bool MULTIBOX = false>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5, 5)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 7)))
 transport_kernel(const KParams P, const KBuffers B)
 {
     // dynamic LDS: the lanes' Philox rings (blockDim.x * 64 B), then in accumulate mode
