@@ -48,7 +48,6 @@ enum Epi : int {
     EPI_BIAS_GELU = 1,    // Z fp32 = acc + bias, H bf16 = gelu(Z)
     EPI_BIAS_F32 = 2,     // C fp32 = acc + bias
     EPI_BIAS_BF16 = 3,    // H bf16 = acc + bias (no activation)
-    EPI_GELU_BWD = 4,     // dZ bf16 = acc * gelu'(Zin)  (Zin fp32, same layout as C)
 };
 
 struct GemmArgs {
@@ -60,27 +59,27 @@ struct GemmArgs {
                                          // product of K slice z goes to C + z * c_slice
     __bf16 *H; int ldh;                  // bf16 output
     const float *bias;                   // [N]
-    const float *Zin;                    // EPI_GELU_BWD: pre-activation, ldc layout
 };
 
+// One 64 x 64 output tile (bx, by) of K slice bz of nz.
 template <int EPI>
-__global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int bx, int by, int bz, int nz)
 {
     __shared__ __bf16 sA[BM][KP + LDS_PAD];        // [m][k]
     __shared__ __bf16 sB[BN][KP + LDS_PAD];        // [n][k]
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int m0 = by * BM, n0 = bx * BN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
 
-    // K slice of this workgroup (gridDim.z slices of K / gridDim.z, a multiple of BK),
-    // staged KP at a time: every load of a phase is in flight at once, then each wave
-    // runs KP/16 MFMA steps on it
-    const int kc = g.K / (int)gridDim.z;
-    const int kbeg = (int)blockIdx.z * kc, kend = kbeg + kc;
+    // K slice of this workgroup (nz slices of K / nz, a multiple of BK), staged KP at a
+    // time: every load of a phase is in flight at once, then each wave runs KP/16 MFMA
+    // steps on it
+    const int kc = g.K / nz;
+    const int kbeg = bz * kc, kend = kbeg + kc;
     for (int k0 = kbeg; k0 < kend; k0 += KP) {
         const int kp = min(KP, kend - k0);         // a multiple of BK
         const int chunks = kp * (BM / 8);          // 8-element chunks per tile (BM == BN)
@@ -132,19 +131,41 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
         const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
         float v = acc[i];
         if (EPI == EPI_F32) {
-            g.C[(size_t)blockIdx.z * g.c_slice + (size_t)row * g.ldc + col] = v;
+            g.C[(size_t)bz * g.c_slice + (size_t)row * g.ldc + col] = v;
         } else if (EPI == EPI_BIAS_F32) {
             g.C[(size_t)row * g.ldc + col] = v + g.bias[col];
         } else if (EPI == EPI_BIAS_GELU) {
             v += g.bias[col];
             if (g.C) g.C[(size_t)row * g.ldc + col] = v;
             g.H[(size_t)row * g.ldh + col] = to_bf16(gelu(v));
-        } else if (EPI == EPI_BIAS_BF16) {
+        } else {  // EPI_BIAS_BF16
             g.H[(size_t)row * g.ldh + col] = to_bf16(v + g.bias[col]);
-        } else {  // EPI_GELU_BWD
-            g.H[(size_t)row * g.ldh + col] = to_bf16(v * gelu_grad(g.Zin[(size_t)row * g.ldc + col]));
         }
     }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
+{
+    gemm_tile<EPI>(g, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z);
+}
+
+// Several independent split-K products in one launch: product j owns the z range
+// [z0[j], z0[j+1]) (its K slices); blocks beyond its tile grid return at once.
+constexpr int MAXL = 9;                             // layers of the MLP, at most
+struct GroupArgs {
+    int n;
+    GemmArgs a[MAXL];
+    int z0[MAXL + 1];
+};
+
+__global__ void __launch_bounds__(256) gemm_grouped(const GroupArgs G)
+{
+    int j = 0;
+    while (j + 1 < G.n && (int)blockIdx.z >= G.z0[j + 1]) ++j;
+    const GemmArgs &g = G.a[j];
+    if ((int)blockIdx.x >= g.N / BN || (int)blockIdx.y >= g.M / BM) return;
+    gemm_tile<EPI_F32>(g, blockIdx.x, blockIdx.y, blockIdx.z - G.z0[j], G.z0[j + 1] - G.z0[j]);
 }
 
 // ---------------------------------------------------------------- small kernels ----
@@ -163,111 +184,313 @@ __global__ void pack_inputs(const float *X, int M, int Mpad, int n_in, int kin_p
     out[i] = to_bf16(v);
 }
 
-// loss and its gradient over the real rows, normalized by 1 / sum w:
-//   columns c <  n_pois: Poisson negative log-likelihood of a log-rate y for an observed
-//                        rate t: w (e^y - t y), gradient w (e^y - t);
-//   columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t).
-__global__ void __launch_bounds__(256) loss_grad(const float *Y, const float *T, const float *W,
-                                                 int M, int Mpad, int N, int n_pois,
-                                                 float inv_wsum, __bf16 *dY, float *loss_part,
-                                                 const int64_t *bidx, int64_t bstride)
+// ------------------------------------------------------------ the training step ----
+// A step is four launches: mlp_rows (inputs, forward, loss, the backward data path and the
+// bias-gradient partials, one workgroup per 32 rows), gemm_grouped (the weight gradients of
+// every layer as split-K products), reduce_grads (the partials summed in a fixed order) and
+// adam_refresh (Adam, the bf16 weight copies).  Every sum runs in a fixed order, so a
+// training run is reproducible.
+
+constexpr int RB = 32;                              // rows per mlp_rows workgroup
+constexpr int RT = 512;                             // its threads: 8 waves, one 32-column block each
+constexpr int AMAX = 256;                           // widest layer (hidden, n_out, kin_pad)
+constexpr int KB = 128;                             // K staged in LDS per phase (mlp_rows)
+
+struct RowArgs {
+    int L, M, n_in, kin_pad, n_out, n_pois;
+    int din[MAXL], dpad[MAXL], dout[MAXL];
+    const __bf16 *Wt[MAXL];                         // [out][in_pad]
+    const __bf16 *Wn[MAXL];                         // [in][out]
+    const float *bias[MAXL];
+    float *Z[MAXL];                                 // hidden layers: pre-activation [Mpad][out]
+    __bf16 *H[MAXL];                                // hidden layers: activation [Mpad][out]
+    __bf16 *dZ[MAXL];                               // gradient of layer l's pre-activation
+    __bf16 *X;                                      // packed inputs [Mpad][kin_pad]
+    float *bpart[MAXL];                             // [n_wg][out] bias-gradient partials
+    float *lpart;                                   // [n_wg] loss partials
+    const float *Xraw, *T, *W, *shift, *scale;
+    float inv_wsum;
+    const int64_t *bidx;                            // batch index (graph replays) or null
+};
+
+// out[RB x N] = A[RB x K] * B^T for B stored [N][ldb] (k contiguous), A resident in LDS.
+// Wave w owns the 32-column block w.  pre(row, col, u, v) loads what the epilogue of an
+// element needs into u, v before the K loop, so those loads complete under the MFMAs;
+// epi(row, col, acc, u, v) stores the element.  B is staged KB columns of K at a time, the
+// next phase's global loads in flight during this phase's MFMAs.
+template <class Pre, class Epi>
+__device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
+                                          const __bf16 *B, int ldb,
+                                          __bf16 (*sB)[KB + LDS_PAD], Pre pre, Epi epi)
 {
-    __shared__ float red[4];
-    if (bidx) {                                     // targets and weights of batch *bidx
-        T += *bidx * bstride;
-        if (W) W += *bidx * bstride;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool mine = wave < N / 32;
+    const int col = wave * 32 + (lane & 31);
+    float pu[16], pv[16];
+    if (mine) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) pre(8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i]);
     }
-    float l = 0.0f;
-    const int64_t tot = (int64_t)Mpad * N;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(i / N), c = (int)(i % N);
-        float g = 0.0f;
-        if (r < M) {
-            const float w = W ? W[i] : 1.0f;
-            if (c < n_pois) {
-                const float e = __expf(Y[i]);
-                l += w * (e - T[i] * Y[i]) * inv_wsum;
-                g = w * (e - T[i]) * inv_wsum;
-            } else {
-                const float d = Y[i] - T[i];
-                l += w * d * d * inv_wsum;
-                g = 2.0f * w * d * inv_wsum;
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    constexpr int CPT = AMAX * KB / 8 / RT;         // 8-element chunks per thread at N = AMAX
+    bf16x8 vb[CPT];
+    // phase k0 covers k0 .. k0 + kp - 1, kp = min(KB, K - k0) (a multiple of 64): N rows
+    // of kp / 8 chunks
+    auto load = [&](int k0) {
+        const int cpr = min(KB, K - k0) / 8, chunks = N * cpr;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * RT;
+            if (c < chunks) vb[u] = *(const bf16x8 *)(B + (size_t)(c / cpr) * ldb + k0 + (c % cpr) * 8);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += KB) {
+        const int kp = min(KB, K - k0);
+        const int cpr = kp / 8, chunks = N * cpr;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * RT;
+            if (c < chunks) *(bf16x8 *)&sB[c / cpr][(c % cpr) * 8] = vb[u];
+        }
+        __syncthreads();
+        if (k0 + KB < K) load(k0 + KB);            // next phase, in flight during the MFMAs
+        if (mine) {
+            for (int kk = 0; kk < kp; kk += 16) {
+                const int r = lane & 31, kq = kk + 8 * (lane >> 5);
+                const bf16x8 a = *(const bf16x8 *)&A[r][k0 + kq];
+                const bf16x8 b = *(const bf16x8 *)&sB[wave * 32 + r][kq];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
         }
-        dY[i] = to_bf16(g);
-    }
-    // fixed-order reduction: the loss is the same on every run
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) l += __shfl_xor(l, m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-// Partial column sums of a bf16 matrix [M x N] (bias gradient): part[s][c] = sum of rows
-// s*rows_per_slice .. of column c.  Block x: 256 columns, block y: slice s.
-__global__ void col_sum_part(const __bf16 *G, int M, int N, int rows_per_slice, float *part)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const int r0 = blockIdx.y * rows_per_slice, r1 = min(M, r0 + rows_per_slice);
-    float s = 0.0f;
-#pragma unroll 8
-    for (int r = r0; r < r1; ++r) s += from_bf16(G[(size_t)r * N + c]);
-    part[(size_t)blockIdx.y * N + c] = s;
-}
-
-// out[r][c] = sum over s = 0..S-1, in order, of P[s * slice + r * ldp + c], for c < cols
-// (split-K partial products, bias partials); out row stride ldo.
-__global__ void sum_slices(const float *P, int S, int64_t slice, int rows, int cols, int ldp,
-                           float *out, int ldo)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)rows * cols;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(i / cols), c = (int)(i % cols);
-        const float *q = P + (size_t)r * ldp + c;
-        float s = 0.0f;
-        int k = 0;
-        for (; k + 8 <= S; k += 8) {              // eight loads in flight, added in order
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = q[(size_t)(k + j) * slice];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) s += v[j];
-        }
-        for (; k < S; ++k) s += q[(size_t)k * slice];
-        out[(size_t)r * ldo + c] = s;
-    }
-}
-
-// Sum of n values by one workgroup in a fixed order, into out[*idx] (out[0] without idx).
-__global__ void __launch_bounds__(256) sum_small(const float *x, int n, float *out,
-                                                 const int64_t *idx)
-{
-    __shared__ float red[256];
-    float s = 0.0f;
-    for (int i = threadIdx.x; i < n; i += 256) s += x[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[idx ? *idx : 0] = red[0];
+    if (mine) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            epi(8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i]);
+    }
 }
 
-// Adam on fp32 master weights; refreshes the bf16 copies in both layouts
-// Wt [out][in_pad] (forward, k-contiguous) and Wn [in][out] (backward data gradient).
-// dstep (optional, device): the steps taken so far; the bias corrections c1, c2 then
-// follow from it on the device (graph replays)
-__global__ void adam_update(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
-                            float b2, float eps, float c1, float c2, float wd, const int64_t *dstep)
+// column sums of the RB rows of act (bf16) into out[c], c < N, rows in order
+__device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD], int N, float *out)
+{
+    for (int c = threadIdx.x; c < N; c += RT) {
+        float s = 0.0f;
+#pragma unroll 8
+        for (int r = 0; r < RB; ++r) s += from_bf16(act[r][c]);
+        out[c] = s;
+    }
+}
+
+// One workgroup per RB rows of the batch:
+//   pack (x - shift) * scale to bf16;
+//   forward: hidden layers Z = A W^T + b (fp32, kept for the backward), H = gelu(Z) (bf16);
+//   loss of the last layer's output y against t with weight w, normalized by 1 / sum w:
+//     columns c <  n_pois: Poisson NLL of a log-rate, w (e^y - t y), gradient w (e^y - t);
+//     columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t);
+//   backward data path: dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1});
+//   bias-gradient partials (column sums of every dZ_l over the RB rows) and the loss partial.
+// Padded rows (>= M) carry zero inputs and a zero gradient.
+__global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
+{
+    __shared__ __bf16 act[2][RB][AMAX + LDS_PAD];
+    __shared__ __bf16 sB[AMAX][KB + LDS_PAD];
+    __shared__ float red[RT / 64];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * RB;
+    const int64_t b = a.bidx ? *a.bidx : 0;
+    const float *Xraw = a.Xraw + b * (int64_t)a.M * a.n_in;
+    const float *T = a.T + b * (int64_t)a.M * a.n_out;
+    const float *W = a.W ? a.W + b * (int64_t)a.M * a.n_out : nullptr;
+
+    for (int i = tid; i < RB * a.kin_pad; i += RT) {
+        const int r = i / a.kin_pad, c = i % a.kin_pad;
+        float v = 0.0f;
+        if (r0 + r < a.M && c < a.n_in) v = (Xraw[(size_t)(r0 + r) * a.n_in + c] - a.shift[c]) * a.scale[c];
+        const __bf16 h = to_bf16(v);
+        act[0][r][c] = h;
+        a.X[(size_t)(r0 + r) * a.kin_pad + c] = h;
+    }
+    __syncthreads();
+
+    int cur = 0;
+    float lsum = 0.0f;
+    for (int l = 0; l < a.L; ++l) {
+        const int N = a.dout[l];
+        __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
+        const float *bias = a.bias[l];
+        if (l + 1 < a.L) {
+            float *Z = a.Z[l];
+            __bf16 *H = a.H[l];
+            rows_gemm(act[cur], a.dpad[l], N, a.Wt[l], a.dpad[l], sB,
+                      [&](int row, int col, float &u, float &) { u = bias[col]; },
+                      [&](int row, int col, float v, float u, float) {
+                          v += u;
+                          const size_t o = (size_t)(r0 + row) * N + col;
+                          Z[o] = v;
+                          const __bf16 h = to_bf16(gelu(v));
+                          H[o] = h;
+                          out[row][col] = h;
+                      });
+        } else {
+            __bf16 *dY = a.dZ[l];
+            const int M = a.M, n_pois = a.n_pois;
+            const float inv_wsum = a.inv_wsum;
+            rows_gemm(act[cur], a.dpad[l], N, a.Wt[l], a.dpad[l], sB,
+                      [&](int row, int col, float &t, float &w) {
+                          const int rg = r0 + row;
+                          const size_t o = (size_t)rg * N + col;
+                          t = rg < M ? T[o] : 0.0f;
+                          w = rg < M ? (W ? W[o] : 1.0f) : 0.0f;
+                      },
+                      [&](int row, int col, float y, float t, float w) {
+                          y += bias[col];
+                          const int rg = r0 + row;
+                          float g = 0.0f;
+                          if (rg < M) {
+                              if (col < n_pois) {
+                                  const float e = __expf(y);
+                                  lsum += w * (e - t * y) * inv_wsum;
+                                  g = w * (e - t) * inv_wsum;
+                              } else {
+                                  const float d = y - t;
+                                  lsum += w * d * d * inv_wsum;
+                                  g = 2.0f * w * d * inv_wsum;
+                              }
+                          }
+                          const __bf16 h = to_bf16(g);
+                          dY[(size_t)rg * N + col] = h;
+                          out[row][col] = h;
+                      });
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    rows_colsum(act[cur], a.dout[a.L - 1], a.bpart[a.L - 1] + (size_t)blockIdx.x * a.dout[a.L - 1]);
+
+    for (int l = a.L - 1; l >= 1; --l) {
+        const int N = a.din[l];                     // = dout[l - 1]
+        const float *Zp = a.Z[l - 1];
+        __bf16 *dZp = a.dZ[l - 1];
+        __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
+        rows_gemm(act[cur], a.dout[l], N, a.Wn[l], a.dout[l], sB,
+                  [&](int row, int col, float &z, float &) { z = Zp[(size_t)(r0 + row) * N + col]; },
+                  [&](int row, int col, float v, float z, float) {
+                      const size_t o = (size_t)(r0 + row) * N + col;
+                      const __bf16 h = to_bf16(v * gelu_grad(z));
+                      dZp[o] = h;
+                      out[row][col] = h;
+                  });
+        __syncthreads();
+        cur ^= 1;
+        rows_colsum(act[cur], N, a.bpart[l - 1] + (size_t)blockIdx.x * N);
+    }
+
+    // loss partial of the workgroup, in a fixed order
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) lsum += __shfl_xor(lsum, m);
+    if ((tid & 63) == 0) red[tid >> 6] = lsum;
+    __syncthreads();
+    if (tid == 0) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < RT / 64; ++w) t += red[w];
+        a.lpart[blockIdx.x] = t;
+    }
+}
+
+struct RedArgs {
+    int L, n_wg;
+    const float *part[MAXL];                        // split-K partials of dW_l, slice stride dout * dpad
+    int ks[MAXL], din[MAXL], dpad[MAXL], dout[MAXL];
+    const float *bpart[MAXL];                       // [n_wg][dout]
+    int64_t woff[MAXL], boff[MAXL], cum[MAXL + 1];  // cum: outputs (dout * din + dout) before layer l
+    float *g;
+    const float *lpart;
+    float *loss;                                    // loss[ctr[0]] with loss_by_batch, else loss[0]
+    int loss_by_batch;
+    int64_t *ctr;                                   // batch index and Adam step, advanced here
+};
+
+// q[0] + q[stride] + ... (n terms), added in order with 16 loads in flight
+__device__ __forceinline__ float sum_strided(const float *q, int n, int64_t stride)
+{
+    float s = 0.0f;
+    int u = 0;
+    for (; u + 16 <= n; u += 16) {
+        float v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = q[(size_t)(u + t) * stride];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) s += v[t];
+    }
+    for (; u < n; ++u) s += q[(size_t)u * stride];
+    return s;
+}
+
+// The gradient: every weight is the in-order sum of its K-slice partials, every bias the
+// in-order sum of the per-workgroup column sums; the loss is the in-order sum of the
+// workgroup partials.  With ctr (graph replays) the batch index and the Adam step count
+// advance here, after the step's last reader of the batch index and before Adam.
+__global__ void __launch_bounds__(256) reduce_grads(const RedArgs a)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.cum[a.L];
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int l = 0;
+        while (i >= a.cum[l + 1]) ++l;
+        const int64_t j = i - a.cum[l];
+        const int64_t nw = (int64_t)a.dout[l] * a.din[l];
+        float s = 0.0f;
+        if (j < nw) {
+            const int o = (int)(j / a.din[l]), k = (int)(j % a.din[l]);
+            const float *q = a.part[l] + (size_t)o * a.dpad[l] + k;
+            const int64_t slice = (int64_t)a.dout[l] * a.dpad[l];
+            int u = 0;
+            for (; u + 8 <= a.ks[l]; u += 8) {      // eight loads in flight, added in order
+                float v[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) v[t] = q[(size_t)(u + t) * slice];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s += v[t];
+            }
+            for (; u < a.ks[l]; ++u) s += q[(size_t)u * slice];
+            a.g[a.woff[l] + j] = s;
+        } else {
+            a.g[a.boff[l] + (j - nw)] = sum_strided(a.bpart[l] + (j - nw), a.n_wg, a.dout[l]);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const float s = sum_strided(a.lpart, a.n_wg, 1);
+        a.loss[a.ctr && a.loss_by_batch ? a.ctr[0] : 0] = s;
+        if (a.ctr) {
+            a.ctr[0] += 1;
+            a.ctr[1] += 1;
+        }
+    }
+}
+
+struct AdamArgs {
+    int L;
+    int64_t woff[MAXL], boff[MAXL];
+    int din[MAXL], dpad[MAXL], dout[MAXL];
+    __bf16 *Wt[MAXL], *Wn[MAXL];
+};
+
+// Adam on the fp32 master weights, then the bf16 copies of every weight in both layouts,
+// Wt [out][in_pad] (forward) and Wn [in][out] (backward data path, layers >= 1); the zero
+// padding of Wt stays as it is.  dstep (graph replays): the Adam step, already advanced;
+// the bias corrections c1, c2 then follow from it on the device.
+__global__ void adam_refresh(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
+                             float b2, float eps, float c1, float c2, float wd, const int64_t *dstep,
+                             const AdamArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (dstep) {
-        const float t = (float)(*dstep + 1);
+        const float t = (float)*dstep;
         c1 = 1.0f / (1.0f - powf(b1, t));
         c2 = 1.0f / (1.0f - powf(b2, t));
     }
@@ -276,13 +499,15 @@ __global__ void adam_update(float *w, const float *g, float *m, float *v, int n,
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
-}
-
-// end of a replayed training step: next batch, one more Adam step
-__global__ void train_step_end(int64_t *ctr)
-{
-    if (threadIdx.x == 0) { ctr[0] += 1; ctr[1] += 1; }
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    int l = 0;
+    while (l + 1 < a.L && (int64_t)i >= a.woff[l + 1]) ++l;
+    if ((int64_t)i >= a.boff[l]) return;            // a bias
+    const int j = i - (int)a.woff[l], o = j / a.din[l], k = j % a.din[l];
+    const __bf16 h = to_bf16(wi);
+    a.Wt[l][(size_t)o * a.dpad[l] + k] = h;
+    if (l > 0) a.Wn[l][(size_t)k * a.dout[l] + o] = h;
 }
 
 __global__ void refresh_bf16(const float *w, int out_dim, int in_dim, int in_pad, __bf16 *Wt,
@@ -433,11 +658,11 @@ struct crysp_mlp {
     float *Y = nullptr;                             // output fp32 [cap][n_out]
     __bf16 *dY = nullptr;
     std::vector<__bf16 *> dZ;                       // [cap][out] per layer
-    __bf16 *dH = nullptr;                           // scratch [cap][hidden]
-    float *part = nullptr;                          // split-K partial weight gradients
-    size_t part_cap = 0;                            // floats
-    float *bpart = nullptr;                         // bias-gradient slice partials
-    float *lpart = nullptr;                         // loss partials [LOSS_BLOCKS]
+    float *part = nullptr;                          // split-K partial weight gradients, all layers
+    std::vector<size_t> part_off;                   // per layer, floats
+    float *bpart = nullptr;                         // bias-gradient partials [n_wg][out], all layers
+    std::vector<size_t> bpart_off;                  // per layer, floats
+    float *lpart = nullptr;                         // loss partials [n_wg]
     float *loss = nullptr;
     int64_t *ctr = nullptr;                         // device: batch index, Adam steps (graph replays)
     int64_t hctr[2] = {0, 0};
@@ -445,9 +670,10 @@ struct crysp_mlp {
 };
 
 // K slices for a weight-gradient product (output M x N, reduction over the batch K): the
-// output alone is only a few 64x64 tiles, so K is split until about 512 workgroups run,
-// keeping each slice a multiple of BK and at least two BK steps.
-constexpr int SPLIT_TARGET_WG = 512;
+// output alone is only a few 64x64 tiles, so K is split until about SPLIT_TARGET_WG
+// workgroups run per layer (all layers share one launch), keeping each slice a multiple of
+// BK and at least two BK steps.
+constexpr int SPLIT_TARGET_WG = 256;
 static int weight_grad_slices(int M, int N, int K)
 {
     const int tiles = (M / BM) * (N / BN);
@@ -456,14 +682,12 @@ static int weight_grad_slices(int M, int N, int K)
         ks *= 2;
     return ks;
 }
-constexpr int LOSS_BLOCKS = 256;                    // loss partials per step
-constexpr int BIAS_ROWS = 32;                       // rows per bias-gradient slice
 
 static int mlp_alloc_batch(crysp_mlp *n, int cap)
 {
     if (cap <= n->cap) return CRYSP_OK;
     auto fre = [](void *q) { if (q) (void)hipFree(q); };
-    fre(n->X); fre(n->Y); fre(n->dY); fre(n->dH);
+    fre(n->X); fre(n->Y); fre(n->dY);
     for (auto q : n->Z) fre(q);
     for (auto q : n->H) fre(q);
     for (auto q : n->dZ) fre(q);
@@ -473,9 +697,10 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
     SCHK(hipMalloc(&n->X, (size_t)cap * n->kin_pad * 2));
     SCHK(hipMalloc(&n->Y, (size_t)cap * n->n_out * 4));
     SCHK(hipMalloc(&n->dY, (size_t)cap * n->n_out * 2));
-    SCHK(hipMalloc(&n->dH, (size_t)cap * std::max(n->hidden, n->n_out) * 2));
-    size_t part = 0;
-    int bias_cols = 0;
+    size_t part = 0, bpart = 0;
+    const int n_wg = (cap + RB - 1) / RB;
+    n->part_off.assign(n->n_layers, 0);
+    n->bpart_off.assign(n->n_layers, 0);
     for (int l = 0; l < n->n_layers; ++l) {
         SCHK(hipMalloc(&n->Z[l], (size_t)cap * n->dout[l] * 4));
         SCHK(hipMalloc(&n->H[l], (size_t)cap * n->dout[l] * 2));
@@ -484,15 +709,16 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
         const int tiles = (n->dout[l] / BM) * (n->dpad[l] / BN);
         int ks_max = 1;
         while (tiles * ks_max * 2 <= SPLIT_TARGET_WG) ks_max *= 2;
-        part = std::max(part, (size_t)ks_max * n->dout[l] * n->dpad[l]);
-        bias_cols = std::max(bias_cols, n->dout[l]);
+        n->part_off[l] = part;
+        part += (size_t)ks_max * n->dout[l] * n->dpad[l];
+        n->bpart_off[l] = bpart;
+        bpart += (size_t)n_wg * n->dout[l];
     }
     fre(n->part); fre(n->bpart); fre(n->lpart);
     n->part = nullptr; n->bpart = nullptr; n->lpart = nullptr;
     SCHK(hipMalloc(&n->part, part * 4));
-    n->part_cap = part;
-    SCHK(hipMalloc(&n->bpart, (size_t)((cap + BIAS_ROWS - 1) / BIAS_ROWS) * bias_cols * 4));
-    SCHK(hipMalloc(&n->lpart, LOSS_BLOCKS * 4));
+    SCHK(hipMalloc(&n->bpart, bpart * 4));
+    SCHK(hipMalloc(&n->lpart, (size_t)n_wg * 4));
     n->cap = cap;
     return CRYSP_OK;
 }
@@ -533,60 +759,84 @@ static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad,
     return CRYSP_OK;
 }
 
-// gradients of the weighted loss into n->g (fp32 flat), the loss into *loss_out (device;
-// n->loss when null)
-static int mlp_backward_dev(crysp_mlp *n, const float *T, const float *W, int M, int Mpad, double wsum,
-                            float *loss_out = nullptr, const int64_t *bidx = nullptr)
+// The gradient of the weighted loss on Mpad rows (M real) into n->g (fp32 flat) and the loss
+// into loss_out[0], or loss_out[ctr[0]] with loss_by_batch.  With ctr (graph replays),
+// ctr[0] is the batch index whose rows are read, and ctr advances.  Three launches:
+// mlp_rows, gemm_grouped (dW of every layer), reduce_grads.
+static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const float *W, int M,
+                        int Mpad, double wsum, float *loss_out, bool loss_by_batch, int64_t *ctr)
 {
-    const int64_t tot = (int64_t)Mpad * n->n_out;
-    const int lblocks = (int)std::min<int64_t>(LOSS_BLOCKS, (tot + 255) / 256);
-    hipLaunchKernelGGL(loss_grad, dim3(lblocks), dim3(256), 0, n->stream, n->Y, T, W, M, Mpad,
-                       n->n_out, n->n_pois, (float)(1.0 / wsum), n->dY, n->lpart, bidx,
-                       (int64_t)M * n->n_out);
-    hipLaunchKernelGGL(sum_small, dim3(1), dim3(256), 0, n->stream, n->lpart, lblocks,
-                       loss_out ? loss_out : n->loss, loss_out ? bidx : nullptr);
-    const __bf16 *dcur = n->dY;                    // gradient w.r.t. layer l's output (pre-act)
-    for (int l = n->n_layers - 1; l >= 0; --l) {
-        const __bf16 *hin = l == 0 ? n->X : n->H[l - 1];
-        const int kin = n->dpad[l], nout = n->dout[l];
-        // dW[out][in] = dZ^T[out x M] * Hin[M x in]   (A = dZ stored [M][out] -> a_t), as
-        // split-K partial products summed in a fixed order; at layer 0 only the real
-        // input columns of the padded width are kept
-        {
-            const int ks = weight_grad_slices(nout, kin, Mpad);
-            if ((size_t)ks * nout * kin > n->part_cap) return sfail(CRYSP_EINVAL, "split-K buffer too small");
-            GemmArgs a{};
-            a.M = nout; a.N = kin; a.K = Mpad;
-            a.A = dcur; a.lda = nout; a.a_t = 1;
-            a.B = hin; a.ldb = kin; a.b_t = 0;
-            a.C = n->part; a.ldc = kin; a.c_slice = (int64_t)nout * kin;
-            gemm<EPI_F32>(n->stream, a, ks);
-            const int64_t cells = (int64_t)nout * n->din[l];
-            hipLaunchKernelGGL(sum_slices, dim3((unsigned)std::min<int64_t>(1024, (cells + 255) / 256)),
-                               dim3(256), 0, n->stream, n->part, ks, a.c_slice, nout, n->din[l], kin,
-                               n->g + n->woff[l], n->din[l]);
-        }
-        {
-            const int slices = (Mpad + BIAS_ROWS - 1) / BIAS_ROWS;
-            hipLaunchKernelGGL(col_sum_part, dim3((nout + 255) / 256, slices), dim3(256), 0,
-                               n->stream, dcur, Mpad, nout, BIAS_ROWS, n->bpart);
-            hipLaunchKernelGGL(sum_slices, dim3((nout + 255) / 256), dim3(256), 0, n->stream,
-                               n->bpart, slices, (int64_t)nout, 1, nout, nout, n->g + n->boff[l], nout);
-        }
-        if (l == 0) break;
-        // dZ_{l-1} = (dZ_l [M x out] * W_l [out x in]) * gelu'(Z_{l-1});  B = Wn [in][out] -> b_t
-        {
-            GemmArgs a{};
-            a.M = Mpad; a.N = n->din[l]; a.K = nout;
-            a.A = dcur; a.lda = nout; a.a_t = 0;
-            a.B = n->Wn[l]; a.ldb = nout; a.b_t = 1;
-            a.H = n->dZ[l - 1]; a.ldh = a.N;
-            a.Zin = n->Z[l - 1]; a.ldc = a.N;
-            gemm<EPI_GELU_BWD>(n->stream, a);
-        }
-        dcur = n->dZ[l - 1];
+    const int L = n->n_layers;
+    if (L > MAXL || Mpad % RB || Mpad > n->cap) return sfail(CRYSP_EINVAL, "bad training shape");
+    const int n_wg = Mpad / RB;
+    RowArgs ra{};
+    ra.L = L; ra.M = M; ra.n_in = n->n_in; ra.kin_pad = n->kin_pad; ra.n_out = n->n_out;
+    ra.n_pois = n->n_pois;
+    for (int l = 0; l < L; ++l) {
+        ra.din[l] = n->din[l]; ra.dpad[l] = n->dpad[l]; ra.dout[l] = n->dout[l];
+        ra.Wt[l] = n->Wt[l]; ra.Wn[l] = n->Wn[l]; ra.bias[l] = n->p + n->boff[l];
+        ra.Z[l] = n->Z[l]; ra.H[l] = n->H[l];
+        ra.dZ[l] = l + 1 == L ? n->dY : n->dZ[l];
+        ra.bpart[l] = n->bpart + n->bpart_off[l];
     }
+    ra.X = n->X; ra.lpart = n->lpart;
+    ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.shift = n->shift; ra.scale = n->scale;
+    ra.inv_wsum = (float)(1.0 / wsum);
+    ra.bidx = ctr;
+    hipLaunchKernelGGL(mlp_rows, dim3(n_wg), dim3(RT), 0, n->stream, ra);
+
+    // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad] (A = dZ stored [M][out]),
+    // every layer in one launch, as split-K partial products
+    GroupArgs ga{};
+    ga.n = L;
+    RedArgs rd{};
+    rd.L = L; rd.n_wg = n_wg;
+    int z = 0;
+    int64_t cum = 0;
+    for (int l = 0; l < L; ++l) {
+        const int kin = n->dpad[l], nout = n->dout[l];
+        const int ks = weight_grad_slices(nout, kin, Mpad);
+        GemmArgs &a = ga.a[l];
+        a.M = nout; a.N = kin; a.K = Mpad;
+        a.A = ra.dZ[l]; a.lda = nout; a.a_t = 1;
+        a.B = l == 0 ? n->X : n->H[l - 1]; a.ldb = kin; a.b_t = 0;
+        a.C = n->part + n->part_off[l]; a.ldc = kin; a.c_slice = (int64_t)nout * kin;
+        ga.z0[l] = z;
+        z += ks;
+        rd.part[l] = a.C; rd.ks[l] = ks;
+        rd.din[l] = n->din[l]; rd.dpad[l] = kin; rd.dout[l] = nout;
+        rd.bpart[l] = ra.bpart[l];
+        rd.woff[l] = (int64_t)n->woff[l]; rd.boff[l] = (int64_t)n->boff[l];
+        rd.cum[l] = cum;
+        cum += (int64_t)nout * n->din[l] + nout;
+    }
+    ga.z0[L] = z;
+    rd.cum[L] = cum;
+    int gx = 1, gy = 1;
+    for (int l = 0; l < L; ++l) {
+        gx = std::max(gx, ga.a[l].N / BN);
+        gy = std::max(gy, ga.a[l].M / BM);
+    }
+    hipLaunchKernelGGL(gemm_grouped, dim3(gx, gy, z), dim3(256), 0, n->stream, ga);
+
+    rd.g = n->g; rd.lpart = n->lpart; rd.loss = loss_out; rd.loss_by_batch = loss_by_batch; rd.ctr = ctr;
+    hipLaunchKernelGGL(reduce_grads, dim3((unsigned)std::min<int64_t>(1024, (cum + 255) / 256)), dim3(256),
+                       0, n->stream, rd);
     return CRYSP_OK;
+}
+
+static void mlp_adam(crysp_mlp *n, float lr, float b1, float b2, float eps, float c1, float c2,
+                     float wd, const int64_t *dstep)
+{
+    AdamArgs aa{};
+    aa.L = n->n_layers;
+    for (int l = 0; l < n->n_layers; ++l) {
+        aa.woff[l] = (int64_t)n->woff[l]; aa.boff[l] = (int64_t)n->boff[l];
+        aa.din[l] = n->din[l]; aa.dpad[l] = n->dpad[l]; aa.dout[l] = n->dout[l];
+        aa.Wt[l] = n->Wt[l]; aa.Wn[l] = n->Wn[l];
+    }
+    hipLaunchKernelGGL(adam_refresh, dim3((unsigned)((n->n_params + 255) / 256)), dim3(256), 0, n->stream,
+                       n->p, n->g, n->m, n->v, (int)n->n_params, lr, b1, b2, eps, c1, c2, wd, dstep, aa);
 }
 
 static void mlp_refresh(crysp_mlp *n)
@@ -630,9 +880,10 @@ const char *crysp_mlp_last_error(void) { return s_err.c_str(); }
 crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_hidden_layers,
                             int32_t n_out, uint64_t seed)
 {
-    if (n_in < 1 || n_in > 32 || hidden < 64 || hidden % 64 || n_out < 64 || n_out % 64 ||
-        n_hidden_layers < 1 || n_hidden_layers > 8) {
-        sfail(CRYSP_EINVAL, "need 1 <= n_in <= 32, hidden and n_out multiples of 64");
+    if (n_in < 1 || n_in > 32 || hidden < 64 || hidden > AMAX || hidden % 64 || n_out < 64 ||
+        n_out > AMAX || n_out % 64 || n_hidden_layers < 1 || n_hidden_layers + 1 > MAXL) {
+        sfail(CRYSP_EINVAL, "need 1 <= n_in <= 32, hidden and n_out multiples of 64 up to 256, "
+                            "1 to 8 hidden layers");
         return nullptr;
     }
     if (hipSetDevice(device) != hipSuccess) { sfail(CRYSP_EHIP, "hipSetDevice"); return nullptr; }
@@ -714,7 +965,7 @@ void crysp_mlp_destroy(crysp_mlp *n)
     fre(n->part); fre(n->bpart); fre(n->lpart);
     for (auto q : n->Wt) fre(q);
     for (auto q : n->Wn) fre(q);
-    fre(n->X); fre(n->Y); fre(n->dY); fre(n->dH);
+    fre(n->X); fre(n->Y); fre(n->dY);
     for (auto q : n->Z) fre(q);
     for (auto q : n->H) fre(q);
     for (auto q : n->dZ) fre(q);
@@ -801,8 +1052,9 @@ int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float
         for (float x : hw) wsum += x;
         if (!(wsum > 0.0)) return sfail(CRYSP_EINVAL, "loss weights sum to zero");
     }
-    if ((rc = mlp_forward_dev(n, (const float *)sx.d, M, mpad))) return rc;
-    if ((rc = mlp_backward_dev(n, (const float *)st.d, (const float *)sw.d, M, mpad, wsum))) return rc;
+    if ((rc = mlp_grad_dev(n, (const float *)sx.d, (const float *)st.d, (const float *)sw.d, M, mpad,
+                           wsum, n->loss, false, nullptr)))
+        return rc;
     if (grads) SCHK(hipMemcpyAsync(grads, n->g, n->n_params * 4, hipMemcpyDeviceToHost, n->stream));
     if (loss) SCHK(hipMemcpyAsync(loss, n->loss, 4, hipMemcpyDeviceToHost, n->stream));
     SCHK(hipStreamSynchronize(n->stream));
@@ -826,19 +1078,15 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
     const double wsum = (double)batch * n->n_out * (W ? mean_row_weight : 1.0f);
     float *dl = nullptr;
     if (losses && n_batches > 0) SCHK(hipMalloc(&dl, (size_t)n_batches * 4));
-    // One step (forward, backward, Adam, bf16 refresh) is captured once and replayed per
-    // batch: the batch index and the Adam step count live on the device and advance at
-    // the end of each replay, so the ~30 small launches of a step cost one graph launch.
-    const unsigned adam_blocks = (unsigned)((n->n_params + 255) / 256);
+    // One step (mlp_rows, the grouped weight gradients, their reduction, Adam with the bf16
+    // refresh) is captured once and replayed per batch: the batch index and the Adam step
+    // count live on the device and advance inside the step, so a step costs one graph launch.
     auto enqueue_step = [&](const float *x, const float *t, const float *w, float *lout,
-                            const int64_t *bidx, const int64_t *dstep, float c1, float c2) -> int {
+                            int64_t *ctr, float c1, float c2) -> int {
         int r;
-        if ((r = mlp_forward_dev(n, x, batch, batch, bidx))) return r;
-        if ((r = mlp_backward_dev(n, t, w, batch, batch, wsum, lout, bidx))) return r;
-        hipLaunchKernelGGL(adam_update, dim3(adam_blocks), dim3(256), 0, n->stream, n->p, n->g,
-                           n->m, n->v, (int)n->n_params, lr, beta1, beta2, eps, c1, c2,
-                           weight_decay, dstep);
-        mlp_refresh(n);
+        if ((r = mlp_grad_dev(n, x, t, w, batch, batch, wsum, lout ? lout : n->loss, lout != nullptr, ctr)))
+            return r;
+        mlp_adam(n, lr, beta1, beta2, eps, c1, c2, weight_decay, ctr ? ctr + 1 : nullptr);
         return CRYSP_OK;
     };
     hipGraphExec_t exec = nullptr;
@@ -849,9 +1097,7 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
         SCHK(hipStreamSynchronize(n->stream));
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            rc = enqueue_step(X, T, W, dl, n->ctr, n->ctr + 1, 0.0f, 0.0f);
-            if (rc == CRYSP_OK)
-                hipLaunchKernelGGL(train_step_end, dim3(1), dim3(64), 0, n->stream, n->ctr);
+            rc = enqueue_step(X, T, W, dl, n->ctr, 0.0f, 0.0f);
             const hipError_t e = hipStreamEndCapture(n->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); if (dl) (void)hipFree(dl); return rc; }
             if (e != hipSuccess || !g || hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
@@ -873,7 +1119,7 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
             n->step += 1;
             const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)n->step));
             const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)n->step));
-            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, nullptr, c1, c2))) {
+            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, c1, c2))) {
                 if (dl) (void)hipFree(dl);
                 return rc;
             }

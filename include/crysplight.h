@@ -276,7 +276,9 @@ int crysp_poisson_samples(uint64_t seed, double lambda, int64_t n, int64_t *out)
 /* ---- learned light-response surrogate (no reference counterpart) ---------------
  * An MLP n_in -> hidden x n_hidden_layers -> n_out with GELU, trained and evaluated by
  * BF16 MFMA GEMMs (v_mfma_f32_32x32x16_bf16) accumulating in FP32, FP32 master weights
- * and Adam.  Parameters are a flat FP32 vector: per layer W [out][in] then b [out]. */
+ * and Adam.  Parameters are a flat FP32 vector: per layer W [out][in] then b [out].
+ * Limits: n_in <= 32; hidden and n_out multiples of 64 up to 256; 1 to 8 hidden layers
+ * (a training step keeps 32 rows of every layer in LDS). */
 typedef struct crysp_mlp crysp_mlp;
 
 crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_hidden_layers,
