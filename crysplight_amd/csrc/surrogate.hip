@@ -197,15 +197,15 @@ constexpr int AMAX = 256;                           // widest layer (hidden, n_o
 constexpr int KB = 128;                             // K staged in LDS per phase (mlp_rows)
 
 struct RowArgs {
-    int L, M, n_in, kin_pad, n_out, n_pois;
+    int L, M, Mpad, n_in, kin_pad, n_out, n_pois;
     int din[MAXL], dpad[MAXL], dout[MAXL];
     const __bf16 *Wt[MAXL];                         // [out][in_pad]
     const __bf16 *Wn[MAXL];                         // [in][out]
     const float *bias[MAXL];
-    float *Z[MAXL];                                 // hidden layers: pre-activation [Mpad][out]
-    __bf16 *H[MAXL];                                // hidden layers: activation [Mpad][out]
-    __bf16 *dZ[MAXL];                               // gradient of layer l's pre-activation
-    __bf16 *X;                                      // packed inputs [Mpad][kin_pad]
+    float *Gp[MAXL];                                // hidden layers: gelu'(pre-activation) [Mpad][out]
+    __bf16 *HT[MAXL];                               // hidden layers: activation, transposed [out][Mpad]
+    __bf16 *dZT[MAXL];                              // gradient of layer l's pre-activation, [out][Mpad]
+    __bf16 *XT;                                     // packed inputs, transposed [kin_pad][Mpad]
     float *bpart[MAXL];                             // [n_wg][out] bias-gradient partials
     float *lpart;                                   // [n_wg] loss partials
     const float *Xraw, *T, *W, *shift, *scale;
@@ -213,15 +213,75 @@ struct RowArgs {
     const int64_t *bidx;                            // batch index (graph replays) or null
 };
 
-// out[RB x N] = A[RB x K] * B^T for B stored [N][ldb] (k contiguous), A resident in LDS.
-// Wave w owns the 32-column block w.  pre(row, col, u, v) loads what the epilogue of an
-// element needs into u, v before the K loop, so those loads complete under the MFMAs;
-// epi(row, col, acc, u, v) stores the element.  B is staged KB columns of K at a time, the
-// next phase's global loads in flight during this phase's MFMAs.
+// The weights of a workgroup's GEMMs, in the order mlp_rows runs them (forward layers
+// 0 .. L-1 with B = Wt[l], then the backward data products of layers L-1 .. 1 with
+// B = Wn[l]), streamed KB columns of K at a time through a register buffer: phase q + 1 is
+// loaded as soon as phase q sits in LDS, so its global loads are in flight during phase q's
+// MFMAs and, at a GEMM boundary, during the epilogue as well.
+struct BStream {
+    constexpr static int CPT = AMAX * KB / 8 / RT;  // 8-element chunks per thread at N = AMAX
+    const RowArgs &a;
+    int Q;                                          // phases in all
+    bf16x8 buf[CPT];
+
+    __device__ __forceinline__ void gemm(int g, const __bf16 *&B, int &ldb, int &N, int &K) const
+    {
+        if (g < a.L) { B = a.Wt[g]; ldb = a.dpad[g]; N = a.dout[g]; K = a.dpad[g]; }
+        else { const int l = 2 * a.L - 1 - g; B = a.Wn[l]; ldb = a.dout[l]; N = a.din[l]; K = a.dout[l]; }
+    }
+    __device__ __forceinline__ BStream(const RowArgs &ra) : a(ra), Q(0)
+    {
+        for (int g = 0; g < 2 * a.L - 1; ++g) {
+            const __bf16 *B; int ldb, N, K;
+            gemm(g, B, ldb, N, K);
+            Q += (K + KB - 1) / KB;
+        }
+    }
+    // phase q: GEMM columns N of B rows, K range k0 .. k0 + kp - 1
+    __device__ __forceinline__ void phase(int q, const __bf16 *&B, int &ldb, int &N, int &k0, int &kp) const
+    {
+        for (int g = 0;; ++g) {
+            int K;
+            gemm(g, B, ldb, N, K);
+            const int np = (K + KB - 1) / KB;
+            if (q < np) { k0 = q * KB; kp = min(KB, K - k0); return; }
+            q -= np;
+        }
+    }
+    // A phase is N rows of kp / 8 chunks of 8 elements (kp = 64 or 128): thread t takes the
+    // chunks t + u * RT, i.e. rows t / cpr + u * (RT / cpr), elements 8 * (t % cpr) onwards.
+    __device__ __forceinline__ static int cpr_shift(int kp) { return kp == KB ? 4 : 3; }
+    __device__ __forceinline__ void load(int q)
+    {
+        if (q >= Q) return;
+        const __bf16 *B; int ldb, N, k0, kp;
+        phase(q, B, ldb, N, k0, kp);
+        const int sh = cpr_shift(kp), tid = threadIdx.x;
+        const int row = tid >> sh, rstep = RT >> sh;
+        const __bf16 *p = B + (size_t)row * ldb + k0 + ((tid & ((1 << sh) - 1)) << 3);
+#pragma unroll
+        for (int u = 0; u < CPT; ++u)
+            if (row + u * rstep < N) buf[u] = *(const bf16x8 *)(p + (size_t)u * rstep * ldb);
+    }
+    // the loaded phase into sB[n][k - k0]
+    __device__ __forceinline__ void stash(int N, int kp, __bf16 (*sB)[KB + LDS_PAD]) const
+    {
+        const int sh = cpr_shift(kp), tid = threadIdx.x;
+        const int row = tid >> sh, rstep = RT >> sh, kk = (tid & ((1 << sh) - 1)) << 3;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u)
+            if (row + u * rstep < N) *(bf16x8 *)&sB[row + u * rstep][kk] = buf[u];
+    }
+};
+
+// out[RB x N] = A[RB x K] * B^T for the next GEMM of the stream (phases q .. q + np - 1),
+// A resident in LDS.  Wave w owns the 32-column block w.  pre(row, col, u, v) loads what
+// the epilogue of an element needs into u, v before the K loop, so those loads complete
+// under the MFMAs; epi(row, col, acc, u, v) stores the element.
 template <class Pre, class Epi>
 __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
-                                          const __bf16 *B, int ldb,
-                                          __bf16 (*sB)[KB + LDS_PAD], Pre pre, Epi epi)
+                                          BStream &bs, int &q, __bf16 (*sB)[KB + LDS_PAD],
+                                          Pre pre, Epi epi)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const bool mine = wave < N / 32;
@@ -234,29 +294,11 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    constexpr int CPT = AMAX * KB / 8 / RT;         // 8-element chunks per thread at N = AMAX
-    bf16x8 vb[CPT];
-    // phase k0 covers k0 .. k0 + kp - 1, kp = min(KB, K - k0) (a multiple of 64): N rows
-    // of kp / 8 chunks
-    auto load = [&](int k0) {
-        const int cpr = min(KB, K - k0) / 8, chunks = N * cpr;
-#pragma unroll
-        for (int u = 0; u < CPT; ++u) {
-            const int c = tid + u * RT;
-            if (c < chunks) vb[u] = *(const bf16x8 *)(B + (size_t)(c / cpr) * ldb + k0 + (c % cpr) * 8);
-        }
-    };
-    load(0);
-    for (int k0 = 0; k0 < K; k0 += KB) {
-        const int kp = min(KB, K - k0);
-        const int cpr = kp / 8, chunks = N * cpr;
-#pragma unroll
-        for (int u = 0; u < CPT; ++u) {
-            const int c = tid + u * RT;
-            if (c < chunks) *(bf16x8 *)&sB[c / cpr][(c % cpr) * 8] = vb[u];
-        }
+    for (int k0 = 0; k0 < K; k0 += KB, ++q) {
+        const int kp = min(KB, K - k0);             // a multiple of 64
+        bs.stash(N, kp, sB);
         __syncthreads();
-        if (k0 + KB < K) load(k0 + KB);            // next phase, in flight during the MFMAs
+        bs.load(q + 1);                             // the next phase, maybe of the next GEMM
         if (mine) {
             for (int kk = 0; kk < kp; kk += 16) {
                 const int r = lane & 31, kq = kk + 8 * (lane >> 5);
@@ -274,6 +316,20 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     }
 }
 
+// The RB rows of act (bf16, N columns) into dst transposed: dst[c * ld + r0 + r].  A
+// thread gathers 8 rows of a column from LDS and stores them as one 16-byte chunk.
+__device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD], int N, __bf16 *dst,
+                                             int64_t ld, int r0)
+{
+    for (int c = threadIdx.x; c < N * (RB / 8); c += RT) {
+        const int col = c / (RB / 8), r = (c % (RB / 8)) * 8;
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = act[r + j][col];
+        *(bf16x8 *)(dst + (size_t)col * ld + r0 + r) = v;
+    }
+}
+
 // column sums of the RB rows of act (bf16) into out[c], c < N, rows in order
 __device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD], int N, float *out)
 {
@@ -287,7 +343,8 @@ __device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD],
 
 // One workgroup per RB rows of the batch:
 //   pack (x - shift) * scale to bf16;
-//   forward: hidden layers Z = A W^T + b (fp32, kept for the backward), H = gelu(Z) (bf16);
+//   forward: hidden layers Z = A W^T + b, H = gelu(Z) (bf16) and gelu'(Z) (fp32, for the
+//   backward);
 //   loss of the last layer's output y against t with weight w, normalized by 1 / sum w:
 //     columns c <  n_pois: Poisson NLL of a log-rate, w (e^y - t y), gradient w (e^y - t);
 //     columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t);
@@ -305,16 +362,18 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
     const float *Xraw = a.Xraw + b * (int64_t)a.M * a.n_in;
     const float *T = a.T + b * (int64_t)a.M * a.n_out;
     const float *W = a.W ? a.W + b * (int64_t)a.M * a.n_out : nullptr;
+    BStream bs(a);
+    bs.load(0);
+    int q = 0;
 
     for (int i = tid; i < RB * a.kin_pad; i += RT) {
         const int r = i / a.kin_pad, c = i % a.kin_pad;
         float v = 0.0f;
         if (r0 + r < a.M && c < a.n_in) v = (Xraw[(size_t)(r0 + r) * a.n_in + c] - a.shift[c]) * a.scale[c];
-        const __bf16 h = to_bf16(v);
-        act[0][r][c] = h;
-        a.X[(size_t)(r0 + r) * a.kin_pad + c] = h;
+        act[0][r][c] = to_bf16(v);
     }
     __syncthreads();
+    rows_store_T(act[0], a.kin_pad, a.XT, a.Mpad, r0);
 
     int cur = 0;
     float lsum = 0.0f;
@@ -323,23 +382,19 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
         const float *bias = a.bias[l];
         if (l + 1 < a.L) {
-            float *Z = a.Z[l];
-            __bf16 *H = a.H[l];
-            rows_gemm(act[cur], a.dpad[l], N, a.Wt[l], a.dpad[l], sB,
+            float *Gp = a.Gp[l];
+            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
                       [&](int row, int col, float &u, float &) { u = bias[col]; },
                       [&](int row, int col, float v, float u, float) {
                           v += u;
                           const size_t o = (size_t)(r0 + row) * N + col;
-                          Z[o] = v;
-                          const __bf16 h = to_bf16(gelu(v));
-                          H[o] = h;
-                          out[row][col] = h;
+                          Gp[o] = gelu_grad(v);
+                          out[row][col] = to_bf16(gelu(v));
                       });
         } else {
-            __bf16 *dY = a.dZ[l];
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
-            rows_gemm(act[cur], a.dpad[l], N, a.Wt[l], a.dpad[l], sB,
+            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
                       [&](int row, int col, float &t, float &w) {
                           const int rg = r0 + row;
                           const size_t o = (size_t)rg * N + col;
@@ -361,31 +416,25 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
                                   g = 2.0f * w * d * inv_wsum;
                               }
                           }
-                          const __bf16 h = to_bf16(g);
-                          dY[(size_t)rg * N + col] = h;
-                          out[row][col] = h;
+                          out[row][col] = to_bf16(g);
                       });
         }
         __syncthreads();
         cur ^= 1;
+        rows_store_T(act[cur], N, l + 1 < a.L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
     }
     rows_colsum(act[cur], a.dout[a.L - 1], a.bpart[a.L - 1] + (size_t)blockIdx.x * a.dout[a.L - 1]);
 
     for (int l = a.L - 1; l >= 1; --l) {
         const int N = a.din[l];                     // = dout[l - 1]
-        const float *Zp = a.Z[l - 1];
-        __bf16 *dZp = a.dZ[l - 1];
+        const float *Gp = a.Gp[l - 1];
         __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
-        rows_gemm(act[cur], a.dout[l], N, a.Wn[l], a.dout[l], sB,
-                  [&](int row, int col, float &z, float &) { z = Zp[(size_t)(r0 + row) * N + col]; },
-                  [&](int row, int col, float v, float z, float) {
-                      const size_t o = (size_t)(r0 + row) * N + col;
-                      const __bf16 h = to_bf16(v * gelu_grad(z));
-                      dZp[o] = h;
-                      out[row][col] = h;
-                  });
+        rows_gemm(act[cur], a.dout[l], N, bs, q, sB,
+                  [&](int row, int col, float &gp, float &) { gp = Gp[(size_t)(r0 + row) * N + col]; },
+                  [&](int row, int col, float v, float gp, float) { out[row][col] = to_bf16(v * gp); });
         __syncthreads();
         cur ^= 1;
+        rows_store_T(act[cur], N, a.dZT[l - 1], a.Mpad, r0);
         rows_colsum(act[cur], N, a.bpart[l - 1] + (size_t)blockIdx.x * N);
     }
 
@@ -653,7 +702,8 @@ struct crysp_mlp {
     // activations for a batch capacity
     int cap = 0;
     __bf16 *X = nullptr;                            // [cap][kin_pad]
-    std::vector<float *> Z;                         // pre-activations fp32 [cap][out]
+    std::vector<float *> Z;                         // fp32 [cap][out]: pre-activations (forward),
+                                                    // their gelu derivative (training step)
     std::vector<__bf16 *> H;                        // post-activations bf16 [cap][out]
     float *Y = nullptr;                             // output fp32 [cap][n_out]
     __bf16 *dY = nullptr;
@@ -770,23 +820,24 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     if (L > MAXL || Mpad % RB || Mpad > n->cap) return sfail(CRYSP_EINVAL, "bad training shape");
     const int n_wg = Mpad / RB;
     RowArgs ra{};
-    ra.L = L; ra.M = M; ra.n_in = n->n_in; ra.kin_pad = n->kin_pad; ra.n_out = n->n_out;
+    ra.L = L; ra.M = M; ra.Mpad = Mpad; ra.n_in = n->n_in; ra.kin_pad = n->kin_pad; ra.n_out = n->n_out;
     ra.n_pois = n->n_pois;
     for (int l = 0; l < L; ++l) {
         ra.din[l] = n->din[l]; ra.dpad[l] = n->dpad[l]; ra.dout[l] = n->dout[l];
         ra.Wt[l] = n->Wt[l]; ra.Wn[l] = n->Wn[l]; ra.bias[l] = n->p + n->boff[l];
-        ra.Z[l] = n->Z[l]; ra.H[l] = n->H[l];
-        ra.dZ[l] = l + 1 == L ? n->dY : n->dZ[l];
+        ra.Gp[l] = n->Z[l]; ra.HT[l] = n->H[l];
+        ra.dZT[l] = l + 1 == L ? n->dY : n->dZ[l];
         ra.bpart[l] = n->bpart + n->bpart_off[l];
     }
-    ra.X = n->X; ra.lpart = n->lpart;
+    ra.XT = n->X; ra.lpart = n->lpart;
     ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.shift = n->shift; ra.scale = n->scale;
     ra.inv_wsum = (float)(1.0 / wsum);
     ra.bidx = ctr;
     hipLaunchKernelGGL(mlp_rows, dim3(n_wg), dim3(RT), 0, n->stream, ra);
 
-    // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad] (A = dZ stored [M][out]),
-    // every layer in one launch, as split-K partial products
+    // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad], both operands stored
+    // transposed (k = the batch row contiguous), every layer in one launch, as split-K
+    // partial products
     GroupArgs ga{};
     ga.n = L;
     RedArgs rd{};
@@ -798,8 +849,8 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
         const int ks = weight_grad_slices(nout, kin, Mpad);
         GemmArgs &a = ga.a[l];
         a.M = nout; a.N = kin; a.K = Mpad;
-        a.A = ra.dZ[l]; a.lda = nout; a.a_t = 1;
-        a.B = l == 0 ? n->X : n->H[l - 1]; a.ldb = kin; a.b_t = 0;
+        a.A = ra.dZT[l]; a.lda = Mpad; a.a_t = 0;
+        a.B = l == 0 ? ra.XT : ra.HT[l - 1]; a.ldb = Mpad; a.b_t = 1;
         a.C = n->part + n->part_off[l]; a.ldc = kin; a.c_slice = (int64_t)nout * kin;
         ga.z0[l] = z;
         z += ks;
