@@ -202,7 +202,6 @@ struct RowArgs {
     const __bf16 *Wt[MAXL];                         // [out][in_pad]
     const __bf16 *Wn[MAXL];                         // [in][out]
     const float *bias[MAXL];
-    float *Gp[MAXL];                                // hidden layers: gelu'(pre-activation) [Mpad][out]
     __bf16 *HT[MAXL];                               // hidden layers: activation, transposed [out][Mpad]
     __bf16 *dZT[MAXL];                              // gradient of layer l's pre-activation, [out][Mpad]
     __bf16 *XT;                                     // packed inputs, transposed [kin_pad][Mpad]
@@ -275,9 +274,10 @@ struct BStream {
 };
 
 // out[RB x N] = A[RB x K] * B^T for the next GEMM of the stream (phases q .. q + np - 1),
-// A resident in LDS.  Wave w owns the 32-column block w.  pre(row, col, u, v) loads what
-// the epilogue of an element needs into u, v before the K loop, so those loads complete
-// under the MFMAs; epi(row, col, acc, u, v) stores the element.
+// A resident in LDS.  Wave w owns the 32-column block w, lane element i (0..15) of it at
+// (row, col).  pre(i, row, col, u, v) loads what the epilogue of an element needs into u, v
+// before the K loop, so those loads complete under the MFMAs; epi(i, row, col, acc, u, v)
+// stores the element.
 template <class Pre, class Epi>
 __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
                                           BStream &bs, int &q, __bf16 (*sB)[KB + LDS_PAD],
@@ -289,7 +289,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     float pu[16], pv[16];
     if (mine) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) pre(8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i]);
+        for (int i = 0; i < 16; ++i) pre(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i]);
     }
     f32x16 acc;
 #pragma unroll
@@ -312,7 +312,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     if (mine) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            epi(8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i]);
+            epi(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i]);
     }
 }
 
@@ -351,6 +351,7 @@ __device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD],
 //   backward data path: dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1});
 //   bias-gradient partials (column sums of every dZ_l over the RB rows) and the loss partial.
 // Padded rows (>= M) carry zero inputs and a zero gradient.
+template <int L>
 __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 {
     __shared__ __bf16 act[2][RB][AMAX + LDS_PAD];
@@ -377,31 +378,33 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 
     int cur = 0;
     float lsum = 0.0f;
-    for (int l = 0; l < a.L; ++l) {
+    // gelu'(Z) of the hidden layers, kept in registers: the backward product of layer l + 1
+    // has the forward layer l's shape, and every element stays with the lane that made it
+    float gpr[L - 1][16];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
         const int N = a.dout[l];
         __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
         const float *bias = a.bias[l];
-        if (l + 1 < a.L) {
-            float *Gp = a.Gp[l];
+        if (l + 1 < L) {
             rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int row, int col, float &u, float &) { u = bias[col]; },
-                      [&](int row, int col, float v, float u, float) {
+                      [&](int, int row, int col, float &u, float &) { u = bias[col]; },
+                      [&](int i, int row, int col, float v, float u, float) {
                           v += u;
-                          const size_t o = (size_t)(r0 + row) * N + col;
-                          Gp[o] = gelu_grad(v);
+                          gpr[l < L - 1 ? l : 0][i] = gelu_grad(v);
                           out[row][col] = to_bf16(gelu(v));
                       });
         } else {
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
             rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int row, int col, float &t, float &w) {
+                      [&](int, int row, int col, float &t, float &w) {
                           const int rg = r0 + row;
                           const size_t o = (size_t)rg * N + col;
                           t = rg < M ? T[o] : 0.0f;
                           w = rg < M ? (W ? W[o] : 1.0f) : 0.0f;
                       },
-                      [&](int row, int col, float y, float t, float w) {
+                      [&](int, int row, int col, float y, float t, float w) {
                           y += bias[col];
                           const int rg = r0 + row;
                           float g = 0.0f;
@@ -421,17 +424,19 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         }
         __syncthreads();
         cur ^= 1;
-        rows_store_T(act[cur], N, l + 1 < a.L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
+        rows_store_T(act[cur], N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
     }
-    rows_colsum(act[cur], a.dout[a.L - 1], a.bpart[a.L - 1] + (size_t)blockIdx.x * a.dout[a.L - 1]);
+    rows_colsum(act[cur], a.dout[L - 1], a.bpart[L - 1] + (size_t)blockIdx.x * a.dout[L - 1]);
 
-    for (int l = a.L - 1; l >= 1; --l) {
+#pragma unroll
+    for (int l = L - 1; l >= 1; --l) {
         const int N = a.din[l];                     // = dout[l - 1]
-        const float *Gp = a.Gp[l - 1];
         __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
         rows_gemm(act[cur], a.dout[l], N, bs, q, sB,
-                  [&](int row, int col, float &gp, float &) { gp = Gp[(size_t)(r0 + row) * N + col]; },
-                  [&](int row, int col, float v, float gp, float) { out[row][col] = to_bf16(v * gp); });
+                  [&](int, int, int, float &, float &) {},
+                  [&](int i, int row, int col, float v, float, float) {
+                      out[row][col] = to_bf16(v * gpr[l - 1][i]);
+                  });
         __syncthreads();
         cur ^= 1;
         rows_store_T(act[cur], N, a.dZT[l - 1], a.Mpad, r0);
@@ -825,7 +830,7 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     for (int l = 0; l < L; ++l) {
         ra.din[l] = n->din[l]; ra.dpad[l] = n->dpad[l]; ra.dout[l] = n->dout[l];
         ra.Wt[l] = n->Wt[l]; ra.Wn[l] = n->Wn[l]; ra.bias[l] = n->p + n->boff[l];
-        ra.Gp[l] = n->Z[l]; ra.HT[l] = n->H[l];
+        ra.HT[l] = n->H[l];
         ra.dZT[l] = l + 1 == L ? n->dY : n->dZ[l];
         ra.bpart[l] = n->bpart + n->bpart_off[l];
     }
@@ -833,7 +838,16 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.shift = n->shift; ra.scale = n->scale;
     ra.inv_wsum = (float)(1.0 / wsum);
     ra.bidx = ctr;
-    hipLaunchKernelGGL(mlp_rows, dim3(n_wg), dim3(RT), 0, n->stream, ra);
+    switch (L) {                                    // the layer count is a template argument, so
+    case 2: hipLaunchKernelGGL(mlp_rows<2>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 3: hipLaunchKernelGGL(mlp_rows<3>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 4: hipLaunchKernelGGL(mlp_rows<4>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 5: hipLaunchKernelGGL(mlp_rows<5>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 6: hipLaunchKernelGGL(mlp_rows<6>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 7: hipLaunchKernelGGL(mlp_rows<7>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    case 8: hipLaunchKernelGGL(mlp_rows<8>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    default: hipLaunchKernelGGL(mlp_rows<9>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
+    }                                               // the layer loops unroll (gelu' in registers)
 
     // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad], both operands stored
     // transposed (k = the batch row contiguous), every layer in one launch, as split-K

@@ -370,11 +370,11 @@ class LightSurrogate:
         T_d = torch.from_numpy(T).to(dev)
         W_d = torch.from_numpy(W).to(dev)
         mrw = float(W.sum(1).mean() / W.shape[1])
-        g = torch.Generator(device="cpu").manual_seed(seed)
+        g = torch.Generator(device=dev).manual_seed(seed)     # permutations drawn on the device
         losses = []
         for ep in range(epochs):
             lr_ep = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
-            perm = torch.randperm(ts.X.shape[0], generator=g)[:n].to(dev)
+            perm = torch.randperm(ts.X.shape[0], generator=g, device=dev)[:n]
             Xb = X_d.index_select(0, perm).contiguous()
             Tb = T_d.index_select(0, perm).contiguous()
             Wb = W_d.index_select(0, perm).contiguous()
