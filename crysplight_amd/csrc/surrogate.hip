@@ -536,13 +536,15 @@ struct AdamArgs {
 // Adam on the fp32 master weights, then the bf16 copies of every weight in both layouts,
 // Wt [out][in_pad] (forward) and Wn [in][out] (backward data path, layers >= 1); the zero
 // padding of Wt stays as it is.  dstep (graph replays): the Adam step, already advanced;
-// the bias corrections c1, c2 then follow from it on the device.
+// the bias corrections c1, c2 then follow from it on the device, and the learning rate is read
+// from dlr.
 __global__ void adam_refresh(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
                              float b2, float eps, float c1, float c2, float wd, const int64_t *dstep,
-                             const AdamArgs a)
+                             const float *dlr, const AdamArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (dlr) lr = *dlr;
     if (dstep) {
         const float t = (float)*dstep;
         c1 = 1.0f / (1.0f - powf(b1, t));
@@ -720,6 +722,15 @@ struct crysp_mlp {
     float *lpart = nullptr;                         // loss partials [n_wg]
     float *loss = nullptr;
     int64_t *ctr = nullptr;                         // device: batch index, Adam steps (graph replays)
+    float *dlr = nullptr;                           // device: learning rate (graph replays)
+    float *dloss = nullptr;                         // device: per-step losses of a call
+    int64_t dloss_cap = 0;
+    // the captured training step, reused while its arguments stay the same
+    hipGraphExec_t gexec = nullptr;
+    const void *gX = nullptr, *gT = nullptr, *gW = nullptr;
+    int gbatch = 0;
+    bool glosses = false;
+    float gb1 = 0, gb2 = 0, geps = 0, gwd = 0, gmrw = 0;
     int64_t hctr[2] = {0, 0};
     double flops_fwd_per_row = 0.0;
 };
@@ -738,9 +749,16 @@ static int weight_grad_slices(int M, int N, int K)
     return ks;
 }
 
+static void mlp_drop_graph(crysp_mlp *n)
+{
+    if (n->gexec) (void)hipGraphExecDestroy(n->gexec);
+    n->gexec = nullptr;
+}
+
 static int mlp_alloc_batch(crysp_mlp *n, int cap)
 {
     if (cap <= n->cap) return CRYSP_OK;
+    mlp_drop_graph(n);                              // it holds the buffers about to be replaced
     auto fre = [](void *q) { if (q) (void)hipFree(q); };
     fre(n->X); fre(n->Y); fre(n->dY);
     for (auto q : n->Z) fre(q);
@@ -891,7 +909,7 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
 }
 
 static void mlp_adam(crysp_mlp *n, float lr, float b1, float b2, float eps, float c1, float c2,
-                     float wd, const int64_t *dstep)
+                     float wd, const int64_t *dstep, const float *dlr)
 {
     AdamArgs aa{};
     aa.L = n->n_layers;
@@ -901,7 +919,7 @@ static void mlp_adam(crysp_mlp *n, float lr, float b1, float b2, float eps, floa
         aa.Wt[l] = n->Wt[l]; aa.Wn[l] = n->Wn[l];
     }
     hipLaunchKernelGGL(adam_refresh, dim3((unsigned)((n->n_params + 255) / 256)), dim3(256), 0, n->stream,
-                       n->p, n->g, n->m, n->v, (int)n->n_params, lr, b1, b2, eps, c1, c2, wd, dstep, aa);
+                       n->p, n->g, n->m, n->v, (int)n->n_params, lr, b1, b2, eps, c1, c2, wd, dstep, dlr, aa);
 }
 
 static void mlp_refresh(crysp_mlp *n)
@@ -973,6 +991,7 @@ crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_
     ok = ok && hipMalloc(&n->shift, 32 * 4) == hipSuccess && hipMalloc(&n->scale, 32 * 4) == hipSuccess;
     ok = ok && hipMalloc(&n->loss, 64) == hipSuccess;
     ok = ok && hipMalloc(&n->ctr, 2 * sizeof(int64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&n->dlr, sizeof(float)) == hipSuccess;
     for (int l = 0; l < n->n_layers && ok; ++l) {
         __bf16 *a = nullptr, *b = nullptr;
         ok = hipMalloc(&a, (size_t)n->dout[l] * n->dpad[l] * 2) == hipSuccess;
@@ -1026,7 +1045,9 @@ void crysp_mlp_destroy(crysp_mlp *n)
     (void)hipSetDevice(n->device);
     if (n->stream) (void)hipStreamSynchronize(n->stream);
     auto fre = [](void *q) { if (q) (void)hipFree(q); };
+    mlp_drop_graph(n);
     fre(n->p); fre(n->g); fre(n->m); fre(n->v); fre(n->shift); fre(n->scale); fre(n->loss); fre(n->ctr);
+    fre(n->dlr); fre(n->dloss);
     fre(n->part); fre(n->bpart); fre(n->lpart);
     for (auto q : n->Wt) fre(q);
     for (auto q : n->Wn) fre(q);
@@ -1069,6 +1090,7 @@ int crysp_mlp_set_input_norm(crysp_mlp *n, const float *shift, const float *scal
 int crysp_mlp_set_poisson_outputs(crysp_mlp *n, int32_t n_pois)
 {
     if (!n || n_pois < 0 || n_pois > n->n_out) return sfail(CRYSP_EINVAL, "bad n_pois");
+    if (n_pois != n->n_pois) mlp_drop_graph(n);     // the captured step holds the old value
     n->n_pois = n_pois;
     return CRYSP_OK;
 }
@@ -1141,41 +1163,59 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
     int rc = mlp_alloc_batch(n, batch);
     if (rc) return rc;
     const double wsum = (double)batch * n->n_out * (W ? mean_row_weight : 1.0f);
-    float *dl = nullptr;
-    if (losses && n_batches > 0) SCHK(hipMalloc(&dl, (size_t)n_batches * 4));
+    if (losses && n_batches > n->dloss_cap) {
+        mlp_drop_graph(n);
+        if (n->dloss) (void)hipFree(n->dloss);
+        n->dloss = nullptr;
+        n->dloss_cap = 0;
+        SCHK(hipMalloc(&n->dloss, (size_t)n_batches * 4));
+        n->dloss_cap = n_batches;
+    }
+    float *dl = losses && n_batches > 0 ? n->dloss : nullptr;
     // One step (mlp_rows, the grouped weight gradients, their reduction, Adam with the bf16
-    // refresh) is captured once and replayed per batch: the batch index and the Adam step
-    // count live on the device and advance inside the step, so a step costs one graph launch.
+    // refresh) is captured once and replayed per batch: the batch index, the Adam step
+    // count and the learning rate live on the device, the first two advancing inside the
+    // step, so a step costs one graph launch.  The captured step is kept for later calls
+    // with the same arrays, batch and optimizer settings (an epoch loop over reshuffled
+    // copies in the same buffers).
     auto enqueue_step = [&](const float *x, const float *t, const float *w, float *lout,
-                            int64_t *ctr, float c1, float c2) -> int {
+                            int64_t *ctr, const float *dlr, float c1, float c2) -> int {
         int r;
         if ((r = mlp_grad_dev(n, x, t, w, batch, batch, wsum, lout ? lout : n->loss, lout != nullptr, ctr)))
             return r;
-        mlp_adam(n, lr, beta1, beta2, eps, c1, c2, weight_decay, ctr ? ctr + 1 : nullptr);
+        mlp_adam(n, lr, beta1, beta2, eps, c1, c2, weight_decay, ctr ? ctr + 1 : nullptr, dlr);
         return CRYSP_OK;
     };
-    hipGraphExec_t exec = nullptr;
+    bool graph = false;
     if (n_batches > 1) {
         n->hctr[0] = 0;
         n->hctr[1] = n->step;
         SCHK(hipMemcpyAsync(n->ctr, n->hctr, sizeof n->hctr, hipMemcpyHostToDevice, n->stream));
-        SCHK(hipStreamSynchronize(n->stream));
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            rc = enqueue_step(X, T, W, dl, n->ctr, 0.0f, 0.0f);
-            const hipError_t e = hipStreamEndCapture(n->stream, &g);
-            if (rc) { if (g) (void)hipGraphDestroy(g); if (dl) (void)hipFree(dl); return rc; }
-            if (e != hipSuccess || !g || hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
-                exec = nullptr;
-            if (g) (void)hipGraphDestroy(g);
+        SCHK(hipMemcpyAsync(n->dlr, &lr, sizeof lr, hipMemcpyHostToDevice, n->stream));
+        const bool same = n->gexec && n->gX == X && n->gT == T && n->gW == W && n->gbatch == batch &&
+                          n->glosses == (dl != nullptr) && n->gb1 == beta1 && n->gb2 == beta2 &&
+                          n->geps == eps && n->gwd == weight_decay && n->gmrw == mean_row_weight;
+        if (!same) {
+            mlp_drop_graph(n);
+            SCHK(hipStreamSynchronize(n->stream));
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                rc = enqueue_step(X, T, W, dl, n->ctr, n->dlr, 0.0f, 0.0f);
+                const hipError_t e = hipStreamEndCapture(n->stream, &g);
+                if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+                if (e != hipSuccess || !g || hipGraphInstantiate(&n->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+                    n->gexec = nullptr;
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (!n->gexec) (void)hipGetLastError();     // capture unavailable: launch eagerly
+            n->gX = X; n->gT = T; n->gW = W; n->gbatch = batch; n->glosses = dl != nullptr;
+            n->gb1 = beta1; n->gb2 = beta2; n->geps = eps; n->gwd = weight_decay; n->gmrw = mean_row_weight;
         }
-        if (!exec) (void)hipGetLastError();         // capture unavailable: launch eagerly
+        graph = n->gexec != nullptr;
     }
-    if (exec) {
-        for (int64_t b = 0; b < n_batches; ++b) SCHK(hipGraphLaunch(exec, n->stream));
+    if (graph) {
+        for (int64_t b = 0; b < n_batches; ++b) SCHK(hipGraphLaunch(n->gexec, n->stream));
         n->step += (int)n_batches;
-        SCHK(hipStreamSynchronize(n->stream));
-        (void)hipGraphExecDestroy(exec);
     } else {
         for (int64_t b = 0; b < n_batches; ++b) {
             const float *x = X + (size_t)b * batch * n->n_in;
@@ -1184,17 +1224,11 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
             n->step += 1;
             const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)n->step));
             const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)n->step));
-            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, c1, c2))) {
-                if (dl) (void)hipFree(dl);
-                return rc;
-            }
+            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, nullptr, c1, c2))) return rc;
         }
     }
-    if (dl) {
-        SCHK(hipMemcpyAsync(losses, dl, (size_t)n_batches * 4, hipMemcpyDeviceToHost, n->stream));
-    }
+    if (dl) SCHK(hipMemcpyAsync(losses, dl, (size_t)n_batches * 4, hipMemcpyDeviceToHost, n->stream));
     SCHK(hipStreamSynchronize(n->stream));
-    if (dl) (void)hipFree(dl);
     return CRYSP_OK;
 }
 

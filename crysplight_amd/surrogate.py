@@ -371,13 +371,18 @@ class LightSurrogate:
         W_d = torch.from_numpy(W).to(dev)
         mrw = float(W.sum(1).mean() / W.shape[1])
         g = torch.Generator(device=dev).manual_seed(seed)     # permutations drawn on the device
+        # shuffled copies in the same buffers every epoch: the library keeps its captured
+        # training step while the arrays stay the same
+        Xb = torch.empty((n, X_d.shape[1]), dtype=X_d.dtype, device=dev)
+        Tb = torch.empty((n, T_d.shape[1]), dtype=T_d.dtype, device=dev)
+        Wb = torch.empty((n, W_d.shape[1]), dtype=W_d.dtype, device=dev)
         losses = []
         for ep in range(epochs):
             lr_ep = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
             perm = torch.randperm(ts.X.shape[0], generator=g, device=dev)[:n]
-            Xb = X_d.index_select(0, perm).contiguous()
-            Tb = T_d.index_select(0, perm).contiguous()
-            Wb = W_d.index_select(0, perm).contiguous()
+            torch.index_select(X_d, 0, perm, out=Xb)
+            torch.index_select(T_d, 0, perm, out=Tb)
+            torch.index_select(W_d, 0, perm, out=Wb)
             torch.cuda.synchronize()
             l = self.net.train_batches(Xb, Tb, Wb, n // batch, batch, lr=lr_ep,
                                        mean_row_weight=mrw)

@@ -151,6 +151,50 @@ def test_training_reduces_loss_on_a_smooth_target():
 
 
 @gpu
+def test_cached_training_step_follows_new_rates_and_contents():
+    """The library keeps its captured training step across calls with the same arrays. A
+    second epoch with a new learning rate and reshuffled contents in the same buffers must
+    give what one eager step per batch gives (same rates, same batches), and a new loss
+    layout (set_poisson_outputs) must retire the captured step."""
+    import torch
+    from crysplight_amd.surrogate import MLP
+    rng = np.random.default_rng(8)
+    B, nb = 2048, 3
+    X = rng.uniform(-1, 1, (nb * B, 4)).astype(np.float32)
+    T = np.concatenate([rng.uniform(0, 0.05, (nb * B, 64)), rng.normal(0, 1, (nb * B, 64))], 1).astype(np.float32)
+    W = rng.uniform(0.5, 2.0, (nb * B, 128)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    perm = rng.permutation(nb * B)
+    epochs = [(X, T, W, 2e-3), (X[perm], T[perm], W[perm], 5e-4)]
+    nets = {}
+    for mode in ("graph", "eager"):
+        net = MLP(4, 256, 3, 128, seed=6)
+        net.set_poisson_outputs(64)
+        Xd, Td, Wd = (torch.empty(a.shape, dtype=torch.float32, device=dev) for a in (X, T, W))
+        losses = []
+        for ex, et, ew, lr in epochs:
+            for d, h in ((Xd, ex), (Td, et), (Wd, ew)):
+                d.copy_(torch.from_numpy(np.ascontiguousarray(h)))
+            if mode == "graph":
+                losses.append(net.train_batches(Xd, Td, Wd, nb, B, lr=lr))
+            else:
+                for b in range(nb):
+                    sl = slice(b * B, (b + 1) * B)
+                    losses.append(net.train_batches(Xd[sl], Td[sl], Wd[sl], 1, B, lr=lr))
+        nets[mode] = (net, np.concatenate(losses), (Xd, Td, Wd))
+    (ng, lg, (Xd, Td, Wd)), (ne, le, _) = nets["graph"], nets["eager"]
+    np.testing.assert_allclose(lg, le, rtol=1e-5)
+    np.testing.assert_allclose(ng.get_params(), ne.get_params(), rtol=1e-4, atol=1e-6)
+    # a new loss layout on the same buffers: the cached step must not be reused
+    ng.set_poisson_outputs(0)
+    ne.set_poisson_outputs(0)
+    a = ng.train_batches(Xd, Td, Wd, nb, B, lr=1e-3)
+    b = np.concatenate([ne.train_batches(Xd[k * B:(k + 1) * B], Td[k * B:(k + 1) * B],
+                                         Wd[k * B:(k + 1) * B], 1, B, lr=1e-3) for k in range(nb)])
+    np.testing.assert_allclose(a, b, rtol=1e-4)
+
+
+@gpu
 def test_surrogate_end_to_end_small():
     """A small surrogate trains, beats its untrained start by a wide margin on held-out MC
     means, and its event maps (with MC fallback) keep the energy sum within a few percent
