@@ -728,7 +728,7 @@ struct crysp_mlp {
     // the captured training step, reused while its arguments stay the same
     hipGraphExec_t gexec = nullptr;
     const void *gX = nullptr, *gT = nullptr, *gW = nullptr;
-    int gbatch = 0;
+    int gbatch = 0, gsteps = 0;                     // batch size, steps captured per graph
     bool glosses = false;
     float gb1 = 0, gb2 = 0, geps = 0, gwd = 0, gmrw = 0;
     int64_t hctr[2] = {0, 0};
@@ -1172,12 +1172,13 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
         n->dloss_cap = n_batches;
     }
     float *dl = losses && n_batches > 0 ? n->dloss : nullptr;
-    // One step (mlp_rows, the grouped weight gradients, their reduction, Adam with the bf16
-    // refresh) is captured once and replayed per batch: the batch index, the Adam step
-    // count and the learning rate live on the device, the first two advancing inside the
-    // step, so a step costs one graph launch.  The captured step is kept for later calls
-    // with the same arrays, batch and optimizer settings (an epoch loop over reshuffled
-    // copies in the same buffers).
+    // A step (mlp_rows, the grouped weight gradients, their reduction, Adam with the bf16
+    // refresh) reads the batch index, the Adam step count and the learning rate from the
+    // device, the first two advancing inside the step, so S identical steps captured in one
+    // graph run S batches in a row (S the largest divisor of n_batches up to 64): one graph
+    // launch per S steps.  The graph is kept for later calls with the same arrays, batch,
+    // step count per graph and optimizer settings (an epoch loop over reshuffled copies in
+    // the same buffers).
     auto enqueue_step = [&](const float *x, const float *t, const float *w, float *lout,
                             int64_t *ctr, const float *dlr, float c1, float c2) -> int {
         int r;
@@ -1187,12 +1188,16 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
         return CRYSP_OK;
     };
     bool graph = false;
+    int steps = 1;
+    for (int d = (int)std::min<int64_t>(64, n_batches); d >= 1; --d)
+        if (n_batches % d == 0) { steps = d; break; }
     if (n_batches > 1) {
         n->hctr[0] = 0;
         n->hctr[1] = n->step;
         SCHK(hipMemcpyAsync(n->ctr, n->hctr, sizeof n->hctr, hipMemcpyHostToDevice, n->stream));
         SCHK(hipMemcpyAsync(n->dlr, &lr, sizeof lr, hipMemcpyHostToDevice, n->stream));
         const bool same = n->gexec && n->gX == X && n->gT == T && n->gW == W && n->gbatch == batch &&
+                          n->gsteps == steps &&
                           n->glosses == (dl != nullptr) && n->gb1 == beta1 && n->gb2 == beta2 &&
                           n->geps == eps && n->gwd == weight_decay && n->gmrw == mean_row_weight;
         if (!same) {
@@ -1200,7 +1205,8 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
             SCHK(hipStreamSynchronize(n->stream));
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                rc = enqueue_step(X, T, W, dl, n->ctr, n->dlr, 0.0f, 0.0f);
+                for (int k = 0; k < steps && rc == CRYSP_OK; ++k)
+                    rc = enqueue_step(X, T, W, dl, n->ctr, n->dlr, 0.0f, 0.0f);
                 const hipError_t e = hipStreamEndCapture(n->stream, &g);
                 if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
                 if (e != hipSuccess || !g || hipGraphInstantiate(&n->gexec, g, nullptr, nullptr, 0) != hipSuccess)
@@ -1208,13 +1214,13 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
                 if (g) (void)hipGraphDestroy(g);
             }
             if (!n->gexec) (void)hipGetLastError();     // capture unavailable: launch eagerly
-            n->gX = X; n->gT = T; n->gW = W; n->gbatch = batch; n->glosses = dl != nullptr;
+            n->gX = X; n->gT = T; n->gW = W; n->gbatch = batch; n->gsteps = steps; n->glosses = dl != nullptr;
             n->gb1 = beta1; n->gb2 = beta2; n->geps = eps; n->gwd = weight_decay; n->gmrw = mean_row_weight;
         }
         graph = n->gexec != nullptr;
     }
     if (graph) {
-        for (int64_t b = 0; b < n_batches; ++b) SCHK(hipGraphLaunch(n->gexec, n->stream));
+        for (int64_t b = 0; b < n_batches; b += steps) SCHK(hipGraphLaunch(n->gexec, n->stream));
         n->step += (int)n_batches;
     } else {
         for (int64_t b = 0; b < n_batches; ++b) {
