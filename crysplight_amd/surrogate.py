@@ -325,31 +325,40 @@ class LightSurrogate:
                                      pid0=pid0, ctx=ctx)
         return TrainingSet(X, counts, tsum, int(photons_per_sample))
 
-    def targets(self, ts: TrainingSet):
+    def targets(self, ts: TrainingSet, device=None):
         """Observed per-photon rates (Poisson outputs, weight 1 / mean rate so gradients
-        read as relative errors) and standardized mean times (weight c / (c + 20))."""
-        p = ts.p
-        tm = ts.tmean
-        counts = ts.counts
+        read as relative errors) and standardized mean times (weight c / (c + 20)), as
+        float32 [n, 2 * ncell] tensors on the device (numpy arrays with device=None)."""
+        import torch
+        dev = torch.device("cpu") if device is None else device
+        cnt = torch.from_numpy(np.ascontiguousarray(ts.counts)).to(dev)
+        tsum = torch.from_numpy(np.ascontiguousarray(ts.tsum)).to(dev)
         if self.fold:
             _, _, code = fold_xy(ts.X[:, 0], ts.X[:, 1], float(self.box.L[0]))
-            p, tm, counts = (to_folded(a, code, self.perms) for a in (p, tm, counts))
-        T = np.empty((ts.X.shape[0], 2 * self.ncell), np.float32)
+            idx = torch.from_numpy(self.perms).to(dev)[torch.from_numpy(code.astype(np.int64)).to(dev)]
+            cnt, tsum = torch.gather(cnt, 1, idx), torch.gather(tsum, 1, idx)
+        p = cnt.double() / float(ts.nph)
+        tm = tsum / cnt.double()                     # 0 / 0 where a cell saw nothing
+        n = ts.X.shape[0]
+        T = torch.empty((n, 2 * self.ncell), dtype=torch.float32, device=dev)
         T[:, :self.ncell] = p
-        T[:, self.ncell:] = np.nan_to_num((tm - self.t_shift) / self.t_scale, nan=0.0)
-        W = np.empty_like(T)
-        self.pois_w = float(1.0 / max(p.mean(), 1e-12))
+        T[:, self.ncell:] = torch.nan_to_num((tm - self.t_shift) / self.t_scale, nan=0.0)
+        W = torch.empty_like(T)
+        self.pois_w = float(1.0 / max(float(p.mean()), 1e-12))
         W[:, :self.ncell] = self.pois_w
-        c = counts.astype(np.float32)
+        c = cnt.float()
         W[:, self.ncell:] = 0.25 * c / (c + 20.0)
-        return T, W
+        return (T.numpy(), W.numpy()) if device is None else (T, W)
 
     def init_output_bias(self, T):
-        """Start the log-rate outputs at the log of each cell's mean rate (T: targets)."""
+        """Start the log-rate outputs at the log of each cell's mean rate (T: targets,
+        numpy or a tensor)."""
         par = self.net.get_params()
         n_out = 2 * self.ncell
         b = par.size - n_out
-        par[b:b + self.ncell] = np.log(np.maximum(T[:, :self.ncell].mean(0), 1e-9))
+        mean = T[:, :self.ncell].mean(0)
+        mean = mean.cpu().numpy() if hasattr(mean, "cpu") else np.asarray(mean)
+        par[b:b + self.ncell] = np.log(np.maximum(mean, 1e-9))
         par[b + self.ncell:] = 0.0
         self.net.set_params(par)
 
@@ -359,17 +368,15 @@ class LightSurrogate:
         """Adam over shuffled mini-batches, learning rate decayed geometrically per epoch.
         Returns the per-step losses."""
         import torch
-        T, W = self.targets(ts)
-        if self.net.get_params()[-2 * self.ncell:].any() == 0:
-            self.init_output_bias(T)
         n = (ts.X.shape[0] // batch) * batch
         if n == 0:
             raise ValueError("training set smaller than one batch")
         dev = torch.device("cuda", self.device)
+        T_d, W_d = self.targets(ts, device=dev)      # prepared on the device
+        if self.net.get_params()[-2 * self.ncell:].any() == 0:
+            self.init_output_bias(T_d)
         X_d = torch.from_numpy(self.features(ts.X[:, :3], ts.X[:, 3])).to(dev)
-        T_d = torch.from_numpy(T).to(dev)
-        W_d = torch.from_numpy(W).to(dev)
-        mrw = float(W.sum(1).mean() / W.shape[1])
+        mrw = float(W_d.sum(1).mean() / W_d.shape[1])
         g = torch.Generator(device=dev).manual_seed(seed)     # permutations drawn on the device
         # shuffled copies in the same buffers every epoch: the library keeps its captured
         # training step while the arrays stay the same

@@ -219,3 +219,31 @@ def test_surrogate_end_to_end_small():
                           tau_ns=1000.0)
     assert abs(r["energy_sum_bias"]) < 0.05, r
     assert r["speedup_sampling_only"] > 10, r
+
+
+def test_training_targets_fold_cells_and_weights():
+    """Targets of a tiny training set: folded cell order (the map of a folded point),
+    per-photon rates, standardized mean times (0 where a cell saw nothing) and weights."""
+    from crysplight_amd import surrogate as S
+    from crysplight_amd.params import Box
+    sur = S.LightSurrogate.__new__(S.LightSurrogate)
+    sur.box, sur.fold, sur.ncell = Box((25.6, 25.6, 20.0)), True, 64
+    sur.perms, sur.t_shift, sur.t_scale = S.fold_perms(8), 100.0, 50.0
+    rng = np.random.default_rng(3)
+    n, nph = 500, 1000
+    X = np.column_stack([rng.uniform(0, 25.6, n), rng.uniform(0, 25.6, n), rng.uniform(0, 20, n),
+                         rng.uniform(10, 511, n)]).astype(np.float32)
+    counts = rng.integers(0, 40, (n, 64)).astype(np.int32)
+    tsum = counts * rng.uniform(50, 400, (n, 64))
+    ts = S.TrainingSet(X, counts, tsum, nph)
+    T, W = sur.targets(ts)
+    _, _, code = S.fold_xy(X[:, 0], X[:, 1], 25.6)
+    cf = S.to_folded(counts, code, sur.perms)
+    tf = S.to_folded(tsum, code, sur.perms)
+    assert np.array_equal(T[:, :64], (cf / nph).astype(np.float32))
+    with np.errstate(invalid="ignore"):
+        want_t = np.nan_to_num((tf / cf - 100.0) / 50.0, nan=0.0).astype(np.float32)
+    assert np.array_equal(T[:, 64:], want_t)
+    assert np.all(T[:, 64:][cf == 0] == 0.0)
+    assert np.allclose(W[:, :64], 1.0 / (counts / nph).mean(), rtol=1e-7)   # float32
+    assert np.array_equal(W[:, 64:], (0.25 * cf.astype(np.float32) / (cf.astype(np.float32) + 20.0)))
