@@ -56,6 +56,8 @@ def main() -> int:
                     help="skip the surrogate training / validation / event comparison")
     ap.add_argument("--surrogate-events", type=int, default=1_000_000)
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=150,
+                    help="training epochs per crystal (fewer only for profiling runs)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the point-source and batched-uneven-events lines")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -282,7 +284,7 @@ def _surrogate(args, ctx):
         box = Box((25.6, 25.6, t))
         sur, info = SE.train_for_box(box, op, ro, tau_ns=tau, tau_rise_ns=rise,
                                      n_samples=args.surrogate_samples, photons=10_000, ctx=ctx,
-                                     seed=11 + int(t))
+                                     seed=11 + int(t), epochs=args.surrogate_epochs)
         surs.append(sur)
         train[f"{t:g}mm"] = info
         val[f"{t:g}mm"] = SE.validate(sur, n_points=1000, photons=2_000_000, seed=77 + int(t),
