@@ -56,7 +56,7 @@ def main() -> int:
                     help="skip the surrogate training / validation / event comparison")
     ap.add_argument("--surrogate-events", type=int, default=1_000_000)
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=150,
+    ap.add_argument("--surrogate-epochs", type=int, default=300,
                     help="training epochs per crystal (fewer only for profiling runs)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the point-source and batched-uneven-events lines")

@@ -26,7 +26,7 @@ def _sync():
 
 
 def train_for_box(box, op, ro, *, tau_ns, tau_rise_ns=0.0, n_samples=1_000_000,
-                  photons=10_000, epochs=150, batch=8192, seed=11, log_depth_mm=1.0, fold=True,
+                  photons=10_000, epochs=300, batch=8192, seed=11, log_depth_mm=1.0, fold=True,
                   ctx=None):
     ctx = ctx or _lib.default_context()
     sur = S.LightSurrogate(box, op, ro, tau_ns=tau_ns, tau_rise_ns=tau_rise_ns, seed=seed,
