@@ -302,7 +302,10 @@ int crysp_mlp_gradient(crysp_mlp *mlp, const float *X, const float *T, const flo
                        int32_t M, float *grads, float *loss);
 /* n_batches Adam steps on consecutive device-resident rows (batch a multiple of 64);
  * mean_row_weight = mean over rows of sum(W) / n_out, the loss normalization.
- * losses[b] (host, optional) receives each step's loss. */
+ * losses[b] (host, optional) receives each step's loss.  A step is four kernels; up to 64
+ * steps are captured in one hipGraph, which is kept for later calls with the same X, T, W
+ * pointers, batch, optimizer settings and loss layout (the learning rate may change: it is
+ * read from device memory).  Refill the same buffers between epochs to reuse it. */
 int crysp_mlp_train_batches(crysp_mlp *mlp, const float *X, const float *T, const float *W,
                             int64_t n_batches, int32_t batch, float lr, float beta1, float beta2,
                             float eps, float weight_decay, float mean_row_weight, float *losses);
