@@ -395,14 +395,14 @@ __device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT
     float sp, cp;
     f_sincos(phi, sp, cp);
     const float cb = st * cp, sb = st * sp;
-    float t1x, t1y, t1z, inv;
-    if (fabsf(uz) < 0.9f) {
-        inv = 1.0f / f_sqrt(ux * ux + uy * uy);
-        t1x = -uy * inv; t1y = ux * inv; t1z = 0.0f;
-    } else {
-        inv = 1.0f / f_sqrt(uy * uy + uz * uz);
-        t1x = 0.0f; t1y = -uz * inv; t1z = uy * inv;
-    }
+    // transverse basis: t1 = (-uy, ux, 0) / |.| if |uz| < 0.9, else (0, -uz, uy) / |.|; the
+    // operands are selected first so that one sqrt and one division serve both cases
+    const bool near_z = fabsf(uz) < 0.9f;
+    const float pa = near_z ? ux : uy, pb = near_z ? uy : uz;
+    float inv = 1.0f / f_sqrt(pa * pa + pb * pb);
+    const float t1x = near_z ? -uy * inv : 0.0f;
+    const float t1y = near_z ? ux * inv : -uz * inv;
+    const float t1z = near_z ? 0.0f : uy * inv;
     const float t2x = uy * t1z - uz * t1y;
     const float t2y = uz * t1x - ux * t1z;
     const float t2z = ux * t1y - uy * t1x;
@@ -518,11 +518,16 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
                                     const float L2)
 {
     oc.add(3);
-    // wall_hit (src/geometry.jl:19-32), ties x -> y -> z
+    // wall_hit (src/geometry.jl:19-32), ties x -> y -> z: (L - x) / u for u > 0, -x / u for
+    // u < 0, Inf for u == 0.  The numerator is selected first so that each axis costs one
+    // division for all lanes, not one per sign branch (a division by zero is discarded).
     const float inf = __builtin_inff();
-    const float dx = p.ux > 0.0f ? (L0 - p.x) / p.ux : (p.ux < 0.0f ? -p.x / p.ux : inf);
-    const float dy = p.uy > 0.0f ? (L1 - p.y) / p.uy : (p.uy < 0.0f ? -p.y / p.uy : inf);
-    const float dz = p.uz > 0.0f ? (L2 - p.z) / p.uz : (p.uz < 0.0f ? -p.z / p.uz : inf);
+    const float qx = (p.ux > 0.0f ? L0 - p.x : -p.x) / p.ux;
+    const float qy = (p.uy > 0.0f ? L1 - p.y : -p.y) / p.uy;
+    const float qz = (p.uz > 0.0f ? L2 - p.z : -p.z) / p.uz;
+    const float dx = p.ux != 0.0f ? qx : inf;
+    const float dy = p.uy != 0.0f ? qy : inf;
+    const float dz = p.uz != 0.0f ? qz : inf;
     float d;
     int face;
     if (dx <= dy && dx <= dz) { d = dx; face = p.ux > 0.0f ? F_XP : F_XM; }
