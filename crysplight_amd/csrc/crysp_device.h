@@ -31,6 +31,7 @@ struct KParams {
     float L0, L1, L2;
     float n_crystal, n_coupling, abs_len, ray_len, wrap_R, sigma_alpha, pde;
     float inv_v;                              // n_crystal / c  [ns/mm]
+    float neg_abs_ln2;                        // -abs_len * ln2 (the absorption estimate)
     float q_coup, q_air;                      // n_crystal / n_coupling, n_crystal / 1
     float cx, cy, r2, sur_R;
     float pitch, bin_ns;
@@ -322,6 +323,10 @@ __device__ __forceinline__ float f_sqrt(float x)
     return s;
 }
 
+// Hardware log2 estimate (v_log_f32), not correctly rounded: only for decisions whose
+// margin covers its error, with the exact f_log behind them.
+__device__ __forceinline__ float f_log2_est(float x) { return __builtin_amdgcn_logf(x); }
+
 // Julia Base semantics used by the reference
 __device__ __forceinline__ float jmax0(float v) { return (v != v) ? v : (v > 0.0f ? v : 0.0f); }
 __device__ __forceinline__ float jclamp(float x, float lo, float hi)
@@ -330,17 +335,54 @@ __device__ __forceinline__ float jclamp(float x, float lo, float hi)
 }
 
 // ----------------------------------------------------------------- optics ----
-// Unpolarized Fresnel with TIR (src/optics.jl:10-19).  Returns true on TIR.  q = n1 / n2
-// is the same binary32 quotient for every call at an interface, divided once on the host.
-__device__ __forceinline__ bool fresnel(float q, float n1, float n2, float cosi, float &R)
+// Unpolarized Fresnel with TIR (src/optics.jl:10-19).  q = n1 / n2 is the same binary32
+// quotient for every call at an interface, divided once on the host.  sin^2 of the
+// refraction angle: TIR when it reaches 1.
+__device__ __forceinline__ float fresnel_sin2t(float q, float cosi)
 {
-    const float sin2t = (q * q) * (1.0f - cosi * cosi);
-    if (sin2t >= 1.0f) { R = 1.0f; return true; }
+    return (q * q) * (1.0f - cosi * cosi);
+}
+
+// The reflectance R for sin2t < 1, every operation rounded as the reference rounds it.
+__device__ __forceinline__ float fresnel_R(float n1, float n2, float cosi, float sin2t)
+{
     const float cost = f_sqrt(1.0f - sin2t);
     const float a = (n1 * cosi - n2 * cost) / (n1 * cosi + n2 * cost);
     const float b = (n1 * cost - n2 * cosi) / (n1 * cost + n2 * cosi);
-    R = 0.5f * (a * a + b * b);
-    return false;
+    return 0.5f * (a * a + b * b);
+}
+
+// The same R from the hardware sqrt and reciprocal estimates (about a third of the
+// instructions).  Each error term of a numerator is a few 2^-24 times a term of its own
+// denominator, so |a|, |b| <= 1 move by a few 2^-23 and R by less than 2^-19, far inside
+// FRESNEL_BAND (the tests pin the gap on a dense cosine grid over several interfaces).
+__device__ __forceinline__ float fresnel_R_est(float n1, float n2, float cosi, float sin2t)
+{
+    const float cost = __builtin_amdgcn_sqrtf(1.0f - sin2t);
+    const float x1 = n1 * cosi, x2 = n2 * cost, y1 = n1 * cost, y2 = n2 * cosi;
+    const float a = (x1 - x2) * __builtin_amdgcn_rcpf(x1 + x2);
+    const float b = (y1 - y2) * __builtin_amdgcn_rcpf(y1 + y2);
+    return 0.5f * (a * a + b * b);
+}
+
+constexpr float FRESNEL_BAND = 6.1035156e-05f;       // 2^-14
+constexpr float ABS_BAND = 2.4414062e-04f;           // 2^-12
+
+// The Fresnel reflection decision at an interface: TIR reflects without a draw;
+// otherwise one draw u reflects when u < R (src/kernel.jl:220-226, 269-274; the UNIFIED
+// tries, src/optics.jl:67-68).  The estimate decides unless u lies within FRESNEL_BAND of
+// it; there the exact R decides.  The outcome is always the exact R's.
+template <bool COUNT>
+__device__ __forceinline__ bool fresnel_reflects(Rng &r, const KParams &P, OpCount<COUNT> &oc,
+                                                 float q, float n1, float n2, float cosi)
+{
+    const float sin2t = fresnel_sin2t(q, cosi);
+    if (sin2t >= 1.0f) return true;
+    const float u = randu<COUNT>(r, P, oc);
+    const float re = fresnel_R_est(n1, n2, cosi, sin2t);
+    if (u < re - FRESNEL_BAND) return true;
+    if (u >= re + FRESNEL_BAND) return false;
+    return u < fresnel_R(n1, n2, cosi, sin2t);
 }
 
 __device__ __forceinline__ int face_axis(int f) { return f <= 2 ? 1 : (f <= 4 ? 2 : 3); }
@@ -474,12 +516,8 @@ __device__ __forceinline__ bool unified_surface(Rng &r, const KParams &P, OpCoun
     for (int k = k0; k < 20; ++k) {
         float nfx, nfy, nfz, cosi;
         if (!facet_try<COUNT>(r, P, oc, ux, uy, uz, a, sgn, nfx, nfy, nfz, cosi)) continue;
-        float R;
         oc.add(4);
-        const bool tir = fresnel(P.q_air, P.n_crystal, 1.0f, cosi, R);
-        bool refl = tir;
-        if (!tir) refl = randu<COUNT>(r, P, oc) < R;
-        if (!refl) return true;
+        if (!fresnel_reflects<COUNT>(r, P, oc, P.q_air, P.n_crystal, 1.0f, cosi)) return true;
         if (facet_reflect(ux, uy, uz, cosi, nfx, nfy, nfz, a, sgn)) return false;
     }
     specular(ux, uy, uz, face);
@@ -534,10 +572,26 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
     else if (dy <= dz)        { d = dy; face = p.uy > 0.0f ? F_YP : F_YM; }
     else                      { d = dz; face = p.uz > 0.0f ? F_BACK : F_FRONT; }
 
-    float d_abs = inf, d_ray = inf;
-    if (P.flags & K_ABS) d_abs = -P.abs_len * f_log(randu<COUNT>(r, P, oc));
+    // bulk absorption (src/kernel.jl:187-197): absorbed when d_abs = -abs_len log(u)
+    // satisfies d_abs < d and d_abs <= d_ray.  d_abs enters nothing but this decision, so
+    // it is taken on the estimate -abs_len ln2 log2_est(u) (relative error below 2^-21 on
+    // every draw value, pinned by the tests) unless the estimate lies within ABS_BAND of
+    // m = min(d, d_ray); there the exact d_abs decides.  The outcome is the exact one's.
+    float u_abs = 1.0f, d_ray = inf;
+    if (P.flags & K_ABS) u_abs = randu<COUNT>(r, P, oc);
     if (P.flags & K_RAY) d_ray = -P.ray_len * f_log(randu<COUNT>(r, P, oc));
-    if (d_abs < d && d_abs <= d_ray) return ST_ABS_BULK;
+    if (P.flags & K_ABS) {
+        const float m = d_ray < d ? d_ray : d;
+        const float est = P.neg_abs_ln2 * f_log2_est(u_abs);
+        bool absorbed;
+        if (est * (1.0f + ABS_BAND) < m) absorbed = true;
+        else if (est * (1.0f - ABS_BAND) > m) absorbed = false;
+        else {
+            const float d_abs = -P.abs_len * f_log(u_abs);
+            absorbed = d_abs < d && d_abs <= d_ray;
+        }
+        if (absorbed) return ST_ABS_BULK;
+    }
     if (d_ray < d) {
         p.x += d_ray * p.ux; p.y += d_ray * p.uy; p.z += d_ray * p.uz;
         p.t += d_ray * P.inv_v;
@@ -583,12 +637,9 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         if (rough) ok = facet_try<COUNT>(r, P, oc, p.ux, p.uy, p.uz, a, sgn, nfx, nfy, nfz, cf);
         bool refl = false;
         if (ok) {
-            float R;
             oc.add(4);
-            const bool tir = fresnel(readout_face ? P.q_coup : P.q_air, P.n_crystal,
-                                     readout_face ? P.n_coupling : 1.0f, cf, R);
-            refl = tir;
-            if (!tir) refl = randu<COUNT>(r, P, oc) < R;
+            refl = fresnel_reflects<COUNT>(r, P, oc, readout_face ? P.q_coup : P.q_air,
+                                           P.n_crystal, readout_face ? P.n_coupling : 1.0f, cf);
         }
         if (readout_face) {
             if (refl) {

@@ -331,6 +331,22 @@ __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids,
     for (int d = 0; d < n_draws; ++d) out[i * n_draws + d] = rng_next<false>(r, P, oc);
 }
 
+// Largest |fresnel_R_est - fresnel_R| at cosine c over a fixed set of interfaces: the
+// crystal/coupling and crystal/air pairs of the workloads, equal indices, and n1 < n2.
+__device__ float fresnel_gap(float c)
+{
+    constexpr float n1s[9] = {2.15f, 2.15f, 1.79f, 1.79f, 1.95f, 1.95f, 1.5f, 1.3f, 1.0f};
+    constexpr float n2s[9] = {1.5f, 1.0f, 1.5f, 1.0f, 1.45f, 1.0f, 1.5f, 1.5f, 2.15f};
+    float g = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const float s2 = fresnel_sin2t(n1s[k] / n2s[k], c);
+        if (s2 < 1.0f)
+            g = fmaxf(g, fabsf(fresnel_R_est(n1s[k], n2s[k], c, s2) - fresnel_R(n1s[k], n2s[k], c, s2)));
+    }
+    return g;
+}
+
 __global__ void debug_math_kernel(int which, const float *x, int64_t n, float *y)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -341,6 +357,8 @@ __global__ void debug_math_kernel(int which, const float *x, int64_t n, float *y
         else if (which == 3) y[i] = f_sqrt(v);
         else if (which == 4) y[i] = f_cos(v);
         else if (which == 5 || which == 6) { f_sincos_tilt(v, s, c); y[i] = which == 5 ? s : c; }
+        else if (which == 7) y[i] = f_log2_est(v);
+        else if (which == 8) y[i] = fresnel_gap(v);
         else { f_sincos(v, s, c); y[i] = which == 1 ? s : c; }
     }
 }
@@ -473,6 +491,7 @@ static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_par
     P.abs_len = op->abs_len_mm; P.ray_len = op->rayleigh_mm;
     P.wrap_R = op->wrap_R; P.sigma_alpha = op->sigma_alpha; P.pde = op->pde;
     P.inv_v = op->n_crystal / C_MM_NS;
+    P.neg_abs_ln2 = -op->abs_len_mm * 0.6931472f;
     P.q_coup = op->n_crystal / op->n_coupling;
     P.q_air = op->n_crystal / 1.0f;
     P.cx = ro->cx; P.cy = ro->cy; P.r2 = ro->r2; P.sur_R = ro->sur_R;
@@ -1107,7 +1126,7 @@ int crysp_debug_draws(crysp_ctx *c, uint64_t seed, uint64_t pid0, int64_t n_pids
 
 int crysp_debug_math(crysp_ctx *c, int32_t which, const float *x, int64_t n, float *y)
 {
-    if (!c || !x || !y || n < 0 || which < 0 || which > 6) return fail(CRYSP_EINVAL, "bad arguments");
+    if (!c || !x || !y || n < 0 || which < 0 || which > 8) return fail(CRYSP_EINVAL, "bad arguments");
     if (n == 0) return CRYSP_OK;
     HIPCHK(hipSetDevice(c->device));
     const float *dx;

@@ -156,9 +156,12 @@ def debug_draws(seed: int, n_pids: int, n_draws: int, pid0: int = 0, ctx=None) -
 
 
 def debug_math(which: str, x: np.ndarray, ctx=None) -> np.ndarray:
-    """Device Float32 log / sin / cos / sqrt as the transport evaluates them."""
+    """Device Float32 log / sin / cos / sqrt as the transport evaluates them, and the
+    hardware log2 estimate of the bulk-absorption decision ("log2_est"), and the largest
+    gap between the estimated and the exact Fresnel reflectance ("fresnel_gap")."""
     ctx = ctx or _lib.default_context()
-    code = {"log": 0, "sin": 1, "cos": 2, "sqrt": 3, "cos1": 4, "sin_tilt": 5, "cos_tilt": 6}[which]
+    code = {"log": 0, "sin": 1, "cos": 2, "sqrt": 3, "cos1": 4, "sin_tilt": 5, "cos_tilt": 6,
+            "log2_est": 7, "fresnel_gap": 8}[which]
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.empty_like(x)
     _lib.check(_lib.load().crysp_debug_math(ctx.handle, code, x.ctypes.data, x.size,

@@ -202,7 +202,10 @@ int crysp_debug_draws(crysp_ctx *ctx, uint64_t seed, uint64_t pid0, int64_t n_pi
 /* Debug: the device Float32 functions the transport uses, applied to n inputs, for
  * the exhaustive checks of their domains.  which: 0 log, 1 sin, 2 cos (both from the
  * sincos path), 3 sqrt, 4 cos (the stand-alone cos of the Box-Muller gaussian), 5 sin,
- * 6 cos (the micro-facet tilt path, for arguments >= 0). */
+ * 6 cos (the micro-facet tilt path, for arguments >= 0), 7 the hardware log2 estimate
+ * behind the bulk-absorption decision (not a correctly rounded function), 8 the largest
+ * gap between the estimated and the exact Fresnel reflectance at cosine x over a fixed set
+ * of interfaces. */
 int crysp_debug_math(crysp_ctx *ctx, int32_t which, const float *x, int64_t n, float *y);
 
 /* Kernel timing: when enabled, every transport launch is bracketed by hipEvents on

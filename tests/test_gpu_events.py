@@ -103,6 +103,20 @@ def test_events_rise_time_uneven(ctx):
     _check(res, _oracle_maps(K.C1_BOX, K.C1_OP, RO8, dep, 5, 1000.0, 20.0, 12345), 300)
 
 
+def test_events_photon_ids_cross_2_32(ctx, bgo_events):
+    """Event-mode stream numbers pid0 + g + 1 run across 2^32 within the launch, under a
+    seed with key bits above 32."""
+    dep, _ = bgo_events
+    sub = E.Deposits(dep.xyzt[:40], dep.nph[:40], dep.event[:40], dep.e_kev[:40],
+                     int(dep.event[39]) + 1)
+    pid0 = 2 ** 32 - int(sub.nph[:20].sum())
+    seed = 2 ** 40 + 7
+    res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, sub, seed=seed, tau_ns=K.C2_TAU,
+                             pid0=pid0, ctx=ctx)
+    _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, sub, seed, K.C2_TAU, pid0=pid0),
+           sub.n_events)
+
+
 def test_events_schedule_independent_and_autotune(ctx, bgo_events):
     dep, _ = bgo_events
     ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)

@@ -36,6 +36,42 @@ def test_log_exhaustive_on_uniform_domain(ctx):
     assert np.array_equal(g, oracle.math("log", u))
 
 
+def test_absorption_estimate_within_band_on_uniform_domain(ctx):
+    """The bulk-absorption decision d_abs < min(d, d_ray) is taken on the estimate
+    est = f32(-abs_len ln2) * log2_est(u) unless est lies within a relative band of 2^-12
+    of min(d, d_ray); inside it the exact d_abs = -abs_len * log(u) decides.  The outcome is
+    the exact one's when |est - d_abs| stays far inside the band: checked here on every
+    draw value k * 2^-24 for several absorption lengths, with the device's own log2
+    estimate and log, and log2_est(1) = 0 exactly."""
+    u = oracle.uniform_domain()
+    h = T.debug_math("log2_est", u, ctx=ctx)
+    lg = T.debug_math("log", u, ctx=ctx)
+    assert u[-1] == 1.0 and h[-1] == 0.0
+    ex = np.log2(u.astype(np.float64))
+    rel_h = np.abs(h[:-1].astype(np.float64) - ex[:-1]) / np.abs(ex[:-1])
+    assert rel_h.max() < 2.0 ** -21, rel_h.max()
+    for abs_len in (300.0, 400.0, 500.0, 0.37, 1.0e6):
+        k = np.float32(-np.float32(abs_len) * np.float32(0.6931472))
+        est = (k * h).astype(np.float64)
+        d_abs = (np.float32(-abs_len) * lg).astype(np.float64)
+        rel = np.abs(est[:-1] - d_abs[:-1]) / d_abs[:-1]
+        assert rel.max() < 2.0 ** -19, (abs_len, rel.max())
+        assert est[-1] == 0.0 and d_abs[-1] == 0.0
+
+
+def test_fresnel_estimate_gap(ctx):
+    """The Fresnel reflection decision u < R is taken on an estimate of R (hardware sqrt
+    and reciprocals) unless u lies within 2^-14 of it, where the exact R decides.  The
+    largest gap between the two, over the crystal/coupling and crystal/air interfaces of
+    the workloads, equal indices and n1 < n2, stays below 2^-19 on every 4th float of
+    [2^-20, 1] and every float of [1/2, 1]."""
+    b = lambda v: int(np.float32(v).view(np.uint32))
+    c = np.concatenate([np.arange(b(2.0 ** -20), b(0.5), 4, dtype=np.uint32),
+                        np.arange(b(0.5), b(1.0) + 1, dtype=np.uint32)]).view(np.float32)
+    g = T.debug_math("fresnel_gap", c, ctx=ctx)
+    assert g.max() < 2.0 ** -19, (g.max(), c[g.argmax()])
+
+
 @pytest.mark.parametrize("fn", ["sin", "cos", "cos1"])
 def test_sincos_exhaustive_on_phase_domain(ctx, fn):
     """sin/cos of phi = 2f0*Float32(pi)*u for every randu value u."""
@@ -115,6 +151,14 @@ def test_records_two_sided_rise_multi_position(ctx):
                                tau_rise=20.0)
     assert frac == 1.0
     assert (g.idx < 0).any() and (g.idx > 0).any()
+
+
+def test_records_photon_ids_cross_2_32_with_64bit_seed(ctx):
+    """Photon ids pid_offset + k run across 2^32 (the Philox counter's high word goes from 0
+    to 1 mid-launch, src/philox.jl:34-40), under a seed with key bits above 32."""
+    g, r, frac = _records_case(ctx, K.C2_BOX, K.C2_OP, K.C1_RO, 20_000, 2 ** 40 + 7,
+                               (12.8, 12.8, 10.0), K.C2_TAU, pid0=2 ** 32 - 10_000)
+    assert frac == 1.0
 
 
 def test_accumulate_equals_reduced_records(ctx):
