@@ -759,6 +759,7 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
 {
     if (cap <= n->cap) return CRYSP_OK;
     mlp_drop_graph(n);                              // it holds the buffers about to be replaced
+    n->cap = 0;                                     // a failed allocation below leaves none
     auto fre = [](void *q) { if (q) (void)hipFree(q); };
     fre(n->X); fre(n->Y); fre(n->dY);
     for (auto q : n->Z) fre(q);

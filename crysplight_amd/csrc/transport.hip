@@ -897,14 +897,17 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     const size_t lds_ev = ring_bytes(c->wg);
     const int grid = grid_for(c, n_hint, c->wg, c->ppt, lds_ev);
 
-    std::vector<unsigned char> key(sizeof P + sizeof B + sizeof(void *) + 6 * sizeof(int64_t));
+    // everything the captured nodes read: parameters, every buffer address (the kernel's
+    // and the outputs finish_events writes), the launch shape and the variant
+    std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 6 * sizeof(int64_t));
     {
         int64_t extra[6] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
                              (int64_t)c->timing, (int64_t)nph_dev };
+        const void *ptrs[3] = { nph_d, dtmin, dnpe };
         unsigned char *q = key.data();
         std::memcpy(q, &P, sizeof P); q += sizeof P;
         std::memcpy(q, &B, sizeof B); q += sizeof B;
-        std::memcpy(q, &nph_d, sizeof(void *)); q += sizeof(void *);
+        std::memcpy(q, ptrs, sizeof ptrs); q += sizeof ptrs;
         std::memcpy(q, extra, sizeof extra);
     }
     // timing events bracket the transport kernel when launched eagerly, and the whole

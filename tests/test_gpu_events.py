@@ -71,6 +71,26 @@ def test_events_device_arrays_graph_replays(ctx, bgo_events):
     assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref2["maps"])
 
 
+def test_events_graph_follows_new_output_arrays(ctx, bgo_events):
+    """A replay with the same inputs and maps array but new first-time and npe arrays must
+    write the new arrays (each call's outputs are part of the captured graph's key)."""
+    import torch
+    dep, _ = bgo_events
+    dev = torch.device("cuda", 0)
+    darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event))
+    nev = dep.n_events
+    maps = torch.zeros((nev, 8, 8), dtype=torch.int32, device=dev)
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    for rep in range(3):
+        tmin = torch.zeros((nev, 8, 8), dtype=torch.float32, device=dev)
+        npe = torch.full((nev,), -1, dtype=torch.int32, device=dev)
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=darr, out=(maps, tmin, npe))
+        assert np.array_equal(tmin.cpu().numpy().reshape(nev, -1), ref["tmin"]), rep
+        assert np.array_equal(npe.cpu().numpy(), ref["npe"]), rep
+        assert np.array_equal(maps.cpu().numpy().reshape(nev, -1), ref["maps"]), rep
+
+
 def test_events_op_counts_match_oracle(ctx, bgo_events):
     """The counting kernel variant in event mode, eager (host arrays) and through the
     captured graph (device arrays, twice), against the oracle's counters of the same
