@@ -21,7 +21,15 @@ enum : int { F_XP = 1, F_XM = 2, F_YP = 3, F_YM = 4, F_BACK = 5, F_FRONT = 6 };
 
 // flag bits of KParams::flags
 enum : uint32_t { K_ABS = 1u, K_RAY = 2u, K_WRAP_SPEC = 4u, K_AIR_GAP = 8u, K_DISC = 16u,
-                  K_SUR_SPEC = 32u, K_TWO_SIDED = 64u };
+                  K_SUR_SPEC = 32u, K_TWO_SIDED = 64u, K_ROUGH = 128u };
+
+// The optics feature set of a kernel variant.  A specialized variant bakes one flag set
+// into its code: its branches fold at compile time, and no flag masks stay live in
+// registers across the step.  OPT_GENERIC reads the flags from KParams at run time.  The
+// host launches a specialization only when the launch's flags equal it exactly.
+constexpr uint32_t OPT_GENERIC = 0xFFFFFFFFu;
+constexpr uint32_t OPT_ESR = K_ABS | K_WRAP_SPEC | K_AIR_GAP | K_ROUGH;   // specular wrap
+constexpr uint32_t OPT_TEFLON = K_ABS | K_AIR_GAP | K_ROUGH;              // Lambertian wrap
 
 constexpr float TWO_PI_F = 6.2831855f;        // 2f0 * Float32(pi): exact product
 constexpr float C_MM_NS = 299.792458f;        // src/transport.jl:4
@@ -48,6 +56,12 @@ struct KParams {
     int32_t refill_min;                       // refill lanes once this many are idle
     int32_t hist_words;                       // nx*ny*(nbins+1)
 };
+
+template <uint32_t OPT>
+__device__ __forceinline__ bool has(const KParams &P, uint32_t bit)
+{
+    return OPT == OPT_GENERIC ? (P.flags & bit) != 0u : (OPT & bit) != 0u;
+}
 
 // ------------------------------------------------------------------ Philox ----
 // Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
@@ -550,7 +564,7 @@ struct Photon {
 // status after the step (ST_FLYING to continue); `lin` gets the detected SiPM index.
 // L0, L1, L2: the crystal's sides (the launch's box, or the photon's own box when
 // deposits of several crystals share one queue).
-template <bool COUNT>
+template <bool COUNT, uint32_t OPT>
 __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount<COUNT> &oc,
                                     int &lin, int &nscat, const float L0, const float L1,
                                     const float L2)
@@ -578,9 +592,9 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
     // every draw value, pinned by the tests) unless the estimate lies within ABS_BAND of
     // m = min(d, d_ray); there the exact d_abs decides.  The outcome is the exact one's.
     float u_abs = 1.0f, d_ray = inf;
-    if (P.flags & K_ABS) u_abs = randu<COUNT>(r, P, oc);
-    if (P.flags & K_RAY) d_ray = -P.ray_len * f_log(randu<COUNT>(r, P, oc));
-    if (P.flags & K_ABS) {
+    if (has<OPT>(P, K_ABS)) u_abs = randu<COUNT>(r, P, oc);
+    if (has<OPT>(P, K_RAY)) d_ray = -P.ray_len * f_log(randu<COUNT>(r, P, oc));
+    if (has<OPT>(P, K_ABS)) {
         const float m = d_ray < d ? d_ray : d;
         const float est = P.neg_abs_ln2 * f_log2_est(u_abs);
         bool absorbed;
@@ -613,14 +627,14 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
     const float cosi = a == 1 ? fabsf(p.ux) : (a == 2 ? fabsf(p.uy) : fabsf(p.uz));
 
     int result = ST_FLYING;
-    const bool readout_face = face == F_BACK || ((P.flags & K_TWO_SIDED) && face == F_FRONT);
+    const bool readout_face = face == F_BACK || (has<OPT>(P, K_TWO_SIDED) && face == F_FRONT);
     // After the optics of the face, at most one more draw decides the photon: the PDE
     // (readout face, inside the active area), the surround reflectivity (readout face,
     // outside it) or the wrap reflectivity (a wrapped face the photon reached).  The three
     // are one draw site with a per-lane threshold.
     int kind = 0;                                   // 1 PDE, 2 surround, 3 wrap
     float thr = 0.0f;
-    if (!readout_face && !(P.flags & K_AIR_GAP)) {
+    if (!readout_face && !has<OPT>(P, K_AIR_GAP)) {
         // front-painted wrap in optical contact (src/kernel.jl:252-262)
         kind = 3;
         thr = P.wrap_R;
@@ -631,7 +645,7 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         // (src/kernel.jl:263-293).  Both Fresnel evaluations (and the first UNIFIED try's)
         // run as one, with per-lane indices and cosine; each lane's draws keep their order.
         const float sgn = face_sign(face);
-        const bool rough = !readout_face && P.sigma_alpha > 0.0f;
+        const bool rough = !readout_face && has<OPT>(P, K_ROUGH);
         float cf = cosi, nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
         bool ok = true;                              // the (first) try reaches Fresnel
         if (rough) ok = facet_try<COUNT>(r, P, oc, p.ux, p.uy, p.uz, a, sgn, nfx, nfy, nfz, cf);
@@ -646,7 +660,7 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
                 specular(p.ux, p.uy, p.uz, face);
             } else {
                 const float ex = p.x - P.cx, ey = p.y - P.cy;
-                const bool inside = !(P.flags & K_DISC) || ex * ex + ey * ey <= P.r2;
+                const bool inside = !has<OPT>(P, K_DISC) || ex * ex + ey * ey <= P.r2;
                 kind = inside ? 1 : 2;
                 thr = inside ? P.pde : P.sur_R;
             }
@@ -682,7 +696,7 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
             return ST_DETECTED;
         }
         if (!pass) return kind == 2 ? ST_ABS_SUR : ST_ABS_WALL;
-        const bool spec = kind == 2 ? (P.flags & K_SUR_SPEC) != 0u : (P.flags & K_WRAP_SPEC) != 0u;
+        const bool spec = kind == 2 ? has<OPT>(P, K_SUR_SPEC) : has<OPT>(P, K_WRAP_SPEC);
         if (spec) specular(p.ux, p.uy, p.uz, face);
         else lambertian<COUNT>(r, P, oc, face, p.ux, p.uy, p.uz);
     }

@@ -82,7 +82,7 @@ __device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep
 
 // Seven waves per SIMD (at most 72 VGPRs): the waves cover the LDS reads of the Philox
 // rings and each other's dependent binary64 chains.
-template <int MODE, bool COUNT, bool MULTIBOX = false>
+template <int MODE, bool COUNT, bool MULTIBOX = false, uint32_t OPT = OPT_GENERIC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 7)))
 transport_kernel(const KParams P, const KBuffers B)
 {
@@ -195,7 +195,7 @@ transport_kernel(const KParams P, const KBuffers B)
         if (alive) {
             rng_prefetch(rng, P.key0, P.key1);      // the step's Philox block, one site
             int lin = 0;
-            const int st = step<COUNT>(p, rng, P, oc, lin, nscat, bL0, bL1, bL2);
+            const int st = step<COUNT, OPT>(p, rng, P, oc, lin, nscat, bL0, bL1, bL2);
             if (st != ST_FLYING) {
                 alive = false;
                 if (MODE == M_RECORDS) {
@@ -507,6 +507,7 @@ static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_par
     if (ro->disc) f |= K_DISC;
     if (ro->sur_specular) f |= K_SUR_SPEC;
     if (ro->two_sided) f |= K_TWO_SIDED;
+    if (op->sigma_alpha > 0.0f) f |= K_ROUGH;
     P.flags = f;
     P.key0 = (uint32_t)seed; P.key1 = (uint32_t)(seed >> 32);
     P.pid0 = pid0;
@@ -526,6 +527,28 @@ static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
     if (lds_bytes) per_cu = std::max(1, std::min<int>(per_cu, (int)(160 * 1024 / lds_bytes)));
     int64_t cap = (int64_t)c->n_cu * per_cu;
     return (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
+}
+
+// The transport kernel for a launch: the specialization whose optics feature set equals the
+// launch's flags (the BGO/ESR and CsI/Teflon workloads), else the generic kernel.  Records
+// mode and the op-counting variant always run the generic kernel.
+template <int MODE, bool MULTIBOX = false>
+static void launch_optics_variant(const KParams &P, const KBuffers &B, dim3 grid, dim3 block,
+                                  size_t lds, hipStream_t stream)
+{
+    if constexpr (MODE != M_RECORDS) {
+        if (P.flags == OPT_ESR) {
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_ESR>), grid, block,
+                               lds, stream, P, B);
+            return;
+        }
+        if (P.flags == OPT_TEFLON) {
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON>), grid, block,
+                               lds, stream, P, B);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX>), grid, block, lds, stream, P, B);
 }
 
 template <int MODE>
@@ -553,8 +576,7 @@ static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_byte
         hipLaunchKernelGGL((transport_kernel<MODE, true>), dim3(grid), dim3(wg), lds_bytes,
                            c->stream, P, B);
     else
-        hipLaunchKernelGGL((transport_kernel<MODE, false>), dim3(grid), dim3(wg), lds_bytes,
-                           c->stream, P, B);
+        launch_optics_variant<MODE>(P, B, dim3(grid), dim3(wg), lds_bytes, c->stream);
     HIPCHK(hipGetLastError());
     if (c->timing) HIPCHK(hipEventRecord(c->ev1, c->stream));
     return CRYSP_OK;
@@ -930,14 +952,12 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             if (multibox)
-                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false, true>), dim3(grid), dim3(c->wg), lds_ev,
-                                   c->stream, P, B);
+                launch_optics_variant<M_EVENTS, true>(P, B, dim3(grid), dim3(c->wg), lds_ev, c->stream);
             else if (c->count_ops)
                 hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), lds_ev,
                                    c->stream, P, B);
             else
-                hipLaunchKernelGGL((transport_kernel<M_EVENTS, false>), dim3(grid), dim3(c->wg), lds_ev,
-                                   c->stream, P, B);
+                launch_optics_variant<M_EVENTS>(P, B, dim3(grid), dim3(c->wg), lds_ev, c->stream);
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev1, c->stream));
         }
         if (nev > 0)

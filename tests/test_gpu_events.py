@@ -6,7 +6,7 @@ import pytest
 import oracle
 from crysplight_amd import events as E
 from crysplight_amd._lib import CryspError
-from crysplight_amd.params import Box, SipmGrid, readout
+from crysplight_amd.params import Box, SipmGrid, deg2rad, readout
 from tests import cases as K
 
 pytestmark = pytest.mark.gpu
@@ -135,6 +135,40 @@ def test_events_photon_ids_cross_2_32(ctx, bgo_events):
                              pid0=pid0, ctx=ctx)
     _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, sub, seed, K.C2_TAU, pid0=pid0),
            sub.n_events)
+
+
+def test_events_generic_and_specialized_optics(ctx):
+    """Event mode runs a specialized kernel when the optics match one of its feature sets
+    (ESR: specular wrap over an air gap, rough; Teflon: Lambertian wrap over an air gap,
+    rough) and the generic kernel otherwise.  Each flavour against the oracle on the same
+    deposits: ESR, Teflon, a smooth ESR wrap (sigma_alpha 0), a front-painted wrap (no air
+    gap) and the all-processes optics with a disc and surround readout and Rayleigh
+    scattering (generic)."""
+    from crysplight_amd.params import OpticalParams
+    rng = np.random.default_rng(21)
+    nd = 30
+    L = np.array(K.C2_BOX.L, np.float32)
+    xyz = (rng.uniform(0.05, 0.95, (nd, 3)) * L).astype(np.float32)
+    xyzt = np.column_stack([xyz, rng.uniform(0, 3, nd)]).astype(np.float32)
+    nph = rng.integers(50, 400, nd).astype(np.int32)
+    ev = np.sort(rng.integers(0, 10, nd)).astype(np.int32)
+    dep = E.Deposits(xyzt, nph, ev, np.zeros(nd, np.float32), 10)
+    s13 = deg2rad(1.3)
+    ro_all = readout(SipmGrid(3.2, 8, 8), center=(12.8, 12.8), radius=11.0, sur_R=0.9)
+    cases = [
+        (K.C2_OP, RO8),                                                              # ESR
+        (OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.98, False, True, s13, 0.3), RO8),  # Teflon
+        (OpticalParams(2.15, 1.5, 300.0, float("inf"), 0.98, True, True, 0.0, 0.3), RO8),   # smooth
+        (OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.95, False, False, s13, 0.3), RO8),  # painted
+        (OpticalParams(1.95, 1.45, 500.0, 400.0, 0.98, False, True, deg2rad(6), 0.4), ro_all),
+    ]
+    for k, (op, ro) in enumerate(cases):
+        res = E.transport_events(K.C2_BOX, op, ro, dep, seed=31, tau_ns=300.0, ctx=ctx)
+        ref = oracle.run_events(K.C2_BOX, op, ro, dep.xyzt, dep.nph, dep.event, dep.n_events,
+                                seed=31, tau_ns=300.0)
+        assert np.array_equal(res.maps.reshape(10, -1), ref["maps"]), k
+        assert np.array_equal(res.tmin.reshape(10, -1), ref["tmin"]), k
+        assert np.array_equal(res.npe, ref["npe"]), k
 
 
 def test_events_schedule_independent_and_autotune(ctx, bgo_events):
