@@ -30,6 +30,7 @@ enum : uint32_t { K_ABS = 1u, K_RAY = 2u, K_WRAP_SPEC = 4u, K_AIR_GAP = 8u, K_DI
 constexpr uint32_t OPT_GENERIC = 0xFFFFFFFFu;
 constexpr uint32_t OPT_ESR = K_ABS | K_WRAP_SPEC | K_AIR_GAP | K_ROUGH;   // specular wrap
 constexpr uint32_t OPT_TEFLON = K_ABS | K_AIR_GAP | K_ROUGH;              // Lambertian wrap
+constexpr uint32_t OPT_TEFLON_RAY = K_ABS | K_RAY | K_AIR_GAP | K_ROUGH;  // + Rayleigh
 
 constexpr float TWO_PI_F = 6.2831855f;        // 2f0 * Float32(pi): exact product
 constexpr float C_MM_NS = 299.792458f;        // src/transport.jl:4

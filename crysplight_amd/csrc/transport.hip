@@ -530,7 +530,8 @@ static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
 }
 
 // The transport kernel for a launch: the specialization whose optics feature set equals the
-// launch's flags (the BGO/ESR and CsI/Teflon workloads), else the generic kernel.  Records
+// launch's flags (ESR or Teflon over an air gap, Teflon with Rayleigh scattering), else the
+// generic kernel.  Records
 // mode and the op-counting variant always run the generic kernel.
 template <int MODE, bool MULTIBOX = false>
 static void launch_optics_variant(const KParams &P, const KBuffers &B, dim3 grid, dim3 block,
@@ -545,6 +546,11 @@ static void launch_optics_variant(const KParams &P, const KBuffers &B, dim3 grid
         if (P.flags == OPT_TEFLON) {
             hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON>), grid, block,
                                lds, stream, P, B);
+            return;
+        }
+        if (P.flags == OPT_TEFLON_RAY) {
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON_RAY>), grid,
+                               block, lds, stream, P, B);
             return;
         }
     }

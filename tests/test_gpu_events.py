@@ -140,10 +140,10 @@ def test_events_photon_ids_cross_2_32(ctx, bgo_events):
 def test_events_generic_and_specialized_optics(ctx):
     """Event mode runs a specialized kernel when the optics match one of its feature sets
     (ESR: specular wrap over an air gap, rough; Teflon: Lambertian wrap over an air gap,
-    rough) and the generic kernel otherwise.  Each flavour against the oracle on the same
-    deposits: ESR, Teflon, a smooth ESR wrap (sigma_alpha 0), a front-painted wrap (no air
-    gap) and the all-processes optics with a disc and surround readout and Rayleigh
-    scattering (generic)."""
+    rough; Teflon with Rayleigh scattering) and the generic kernel otherwise.  Each flavour
+    against the oracle on the same deposits: ESR, Teflon, Teflon with Rayleigh, a smooth ESR
+    wrap (sigma_alpha 0), a front-painted wrap (no air gap) and the all-processes optics with
+    a disc and surround readout (generic)."""
     from crysplight_amd.params import OpticalParams
     rng = np.random.default_rng(21)
     nd = 30
@@ -158,6 +158,7 @@ def test_events_generic_and_specialized_optics(ctx):
     cases = [
         (K.C2_OP, RO8),                                                              # ESR
         (OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.98, False, True, s13, 0.3), RO8),  # Teflon
+        (OpticalParams(1.95, 1.45, 3000.0, 333.3, 0.99, False, True, s13, 1.0), RO8),    # + Rayleigh
         (OpticalParams(2.15, 1.5, 300.0, float("inf"), 0.98, True, True, 0.0, 0.3), RO8),   # smooth
         (OpticalParams(1.79, 1.5, 400.0, float("inf"), 0.95, False, False, s13, 0.3), RO8),  # painted
         (OpticalParams(1.95, 1.45, 500.0, 400.0, 0.98, False, True, deg2rad(6), 0.4), ro_all),

@@ -173,6 +173,24 @@ def test_accumulate_equals_reduced_records(ctx):
     assert acc.total_terminated() == 200_000
 
 
+@pytest.mark.parametrize("spec,ray", [(True, float("inf")), (False, float("inf")), (False, 333.3)])
+def test_accumulate_specialized_optics_equal_oracle(ctx, spec, ray):
+    """Accumulate mode on the optics feature sets that run specialized kernels (specular or
+    Lambertian wrap over an air gap, rough; Lambertian with Rayleigh scattering, the
+    reference's benchmark optics of scripts/bench_metal.jl:7-13): histogram, first times and
+    counters identical to the oracle's reduction of its records."""
+    from crysplight_amd.params import OpticalParams
+    op = OpticalParams(1.95, 1.45, 3000.0, ray, 0.99, spec, True, deg2rad(1.3), 1.0)
+    ro = readout(SipmGrid(6.0, 8, 8))
+    kw = dict(n_photons=100_000, seed=1, pos=(24.0, 24.0, 18.6), tau_ns=0.0)
+    acc = T.transport_accumulate(K.BOX_CRYSP, op, ro, K.TB, ctx=ctx, **kw)
+    r = oracle.run_photons(K.BOX_CRYSP, op, ro, K.TB, **kw)
+    assert np.array_equal(acc.counts, r["counts"])
+    assert np.array_equal(acc.first_ns, r["first_ns"])
+    for name in ("ndet", "nabs_bulk", "nabs_wall", "nabs_sipm", "ncap", "nscat"):
+        assert getattr(acc, name) == r[name], name
+
+
 def test_op_counts_match_oracle(ctx):
     kw = dict(n_photons=50_000, seed=1234, pos=K.C1_POS, tau_ns=K.C1_TAU)
     ctx.set_op_counting(True)
