@@ -266,20 +266,31 @@ def autotune(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_n
 
 def transport_events_multibox(boxes, dep_box, op: OpticalParams, ro, dep: Deposits, *, seed: int,
                               tau_ns: float, tau_rise_ns: float = 0.0, pid0: int = 0,
-                              max_bounces: int = 100_000, ctx=None) -> EventMaps:
+                              max_bounces: int = 100_000, ctx=None, device_arrays=None,
+                              out=None) -> EventMaps:
     """Event mode with deposits in several crystals in one queue: deposit d lies in
-    boxes[dep_box[d]]."""
+    boxes[dep_box[d]].  device_arrays: optional (xyzt, nph, event, dep_box) torch tensors
+    already on the GPU (dep_box is then ignored); out: optional (maps, tmin, npe) torch
+    tensors on the GPU (then returned as-is)."""
     ctx = ctx or _lib.default_context()
     ro = readout(ro) if isinstance(ro, SipmGrid) else ro
     nev = int(dep.n_events)
     arr = (Box * len(boxes))(*[Box(tuple(b.L)) for b in boxes])
-    db = np.ascontiguousarray(dep_box, np.int32)
-    xyzt = np.ascontiguousarray(dep.xyzt, np.float32)
-    nph = np.ascontiguousarray(dep.nph, np.int32)
-    ev = np.ascontiguousarray(dep.event, np.int32)
-    maps = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.int32)
-    tmin = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.float32)
-    npe = np.zeros(nev, np.int32)
+    if device_arrays is not None:
+        xyzt, nph, ev, db = device_arrays
+        ndep = int(xyzt.shape[0])
+    else:
+        db = np.ascontiguousarray(dep_box, np.int32)
+        xyzt = np.ascontiguousarray(dep.xyzt, np.float32)
+        nph = np.ascontiguousarray(dep.nph, np.int32)
+        ev = np.ascontiguousarray(dep.event, np.int32)
+        ndep = int(xyzt.shape[0])
+    if out is not None:
+        maps, tmin, npe = out
+    else:
+        maps = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.int32)
+        tmin = np.zeros((nev, ro.grid.ny, ro.grid.nx), np.float32)
+        npe = np.zeros(nev, np.int32)
     em = Emission(tau_ns, tau_rise_ns)
     ntot = C.c_int64()
     lib = _lib.load()
@@ -288,10 +299,10 @@ def transport_events_multibox(boxes, dep_box, op: OpticalParams, ro, dep: Deposi
                   C.POINTER(Readout), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                   C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(Emission), C.c_void_p, C.c_void_p,
                   C.c_void_p, C.c_void_p]
-    rc = f(ctx.handle, C.addressof(arr), len(boxes), db.ctypes.data, C.byref(op), C.byref(ro),
-           xyzt.ctypes.data, nph.ctypes.data, ev.ctypes.data, xyzt.shape[0], nev, int(seed),
-           int(pid0), int(max_bounces), C.byref(em), maps.ctypes.data, tmin.ctypes.data,
-           npe.ctypes.data, C.byref(ntot))
+    rc = f(ctx.handle, C.addressof(arr), len(boxes), _lib.ptr(db), C.byref(op), C.byref(ro),
+           _lib.ptr(xyzt), _lib.ptr(nph), _lib.ptr(ev), ndep, nev, int(seed), int(pid0),
+           int(max_bounces), C.byref(em), _lib.ptr(maps), _lib.ptr(tmin), _lib.ptr(npe),
+           C.byref(ntot))
     _lib.check(rc, "crysp_run_events_multibox")
     return EventMaps(maps, tmin, npe, ntot.value)
 

@@ -110,27 +110,43 @@ def gamma_mix(thicknesses, footprint, mat, yield_per_mev, n_events, seed, max_de
 def compare_events(surrogates, boxes, op, ro, dep, part_of, *, yield_per_mev, tau_ns,
                    tau_rise_ns=0.0, seed=5, fallback_mm=1.0, grid_shape=(64, 64), ctx=None):
     """(a) plain MC of every deposit in one multi-crystal queue; (b) surrogate Poisson maps
-    for deposits at least fallback_mm from every face, MC for the rest."""
+    for deposits at least fallback_mm from every face, MC for the rest.  Both paths start
+    from deposit tables resident on the device and leave their maps there (the maps are
+    copied back after the timed regions), like the transport's photons/s."""
     ctx = ctx or _lib.default_context()
     nev = dep.n_events
+    import torch
+    dev = torch.device("cuda", 0)
+    Ls = np.array([b.L for b in boxes], np.float32)[part_of]
+    x = dep.xyzt[:, :3]
+    use = np.all((x >= fallback_mm) & (x <= Ls - fallback_mm), axis=1)
+
+    def on_dev(d, part):
+        return tuple(torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                     for a in (d.xyzt, d.nph, d.event, part.astype(np.int32)))
+
+    grid = ro.grid if hasattr(ro, "grid") else ro
+
+    def map_out():
+        return (torch.zeros((nev, grid.ny, grid.nx), dtype=torch.int32, device=dev),
+                torch.zeros((nev, grid.ny, grid.nx), dtype=torch.float32, device=dev),
+                torch.zeros(nev, dtype=torch.int32, device=dev))
+
+    mc_in, mc_out = on_dev(dep, part_of), map_out()
     # (a) plain MC
     _sync()
     t0 = time.perf_counter()
     mc = transport_events_multibox(boxes, part_of, op, ro, dep, seed=seed, tau_ns=tau_ns,
-                                   tau_rise_ns=tau_rise_ns, ctx=ctx)
+                                   tau_rise_ns=tau_rise_ns, ctx=ctx, device_arrays=mc_in,
+                                   out=mc_out)
     _sync()
     t_mc = time.perf_counter() - t0
     # (b) surrogate + fallback
-    Ls = np.array([b.L for b in boxes], np.float32)[part_of]
-    x = dep.xyzt[:, :3]
-    use = np.all((x >= fallback_mm) & (x <= Ls - fallback_mm), axis=1)
     grids = [s.bake((grid_shape[0], grid_shape[1], max(8, int(round(b.L[2] * 2)) + 1)))
              for s, b in zip(surrogates, boxes)]
     ev_first = np.zeros(nev + 1, np.int64)
     np.add.at(ev_first, dep.event + 1, 1)
     ev_first = np.cumsum(ev_first)
-    import torch
-    dev = torch.device("cuda", 0)
     g_d = [torch.from_numpy(g).to(dev) for g in grids]
     xyzt_d = torch.from_numpy(dep.xyzt).to(dev)
     ek_d = torch.from_numpy(dep.e_kev).to(dev)
@@ -140,6 +156,7 @@ def compare_events(surrogates, boxes, op, ro, dep, part_of, *, yield_per_mev, ta
     maps_d = [torch.zeros((nev, 64), dtype=torch.int32, device=dev) for _ in boxes]
     tm_d = [torch.zeros((nev, 64), dtype=torch.float32, device=dev) for _ in boxes]
     fb = Deposits(dep.xyzt[~use], dep.nph[~use], dep.event[~use], dep.e_kev[~use], nev)
+    fb_in, fb_out = on_dev(fb, part_of[~use]), map_out()
     _sync()
     t0 = time.perf_counter()
     for i, b in enumerate(boxes):
@@ -147,16 +164,21 @@ def compare_events(surrogates, boxes, op, ro, dep, part_of, *, yield_per_mev, ta
                         yield_per_mev, seed=seed + 17 * i, out=(maps_d[i], tm_d[i]))
     _sync()
     t_sur_only = time.perf_counter() - t0
-    fbm = transport_events_multibox(boxes, part_of[~use], op, ro, fb, seed=seed + 99,
-                                    tau_ns=tau_ns, tau_rise_ns=tau_rise_ns, pid0=1 << 40, ctx=ctx)
+    transport_events_multibox(boxes, part_of[~use], op, ro, fb, seed=seed + 99, tau_ns=tau_ns,
+                              tau_rise_ns=tau_rise_ns, pid0=1 << 40, ctx=ctx,
+                              device_arrays=fb_in, out=fb_out)
+    sur_d = fb_out[0].reshape(nev, -1)
+    for m in maps_d:
+        sur_d = sur_d + m
     _sync()
     t_sur = time.perf_counter() - t0
-    sur_maps = sum(m.cpu().numpy() for m in maps_d) + fbm.maps.reshape(nev, -1)
-    mc_maps = mc.maps.reshape(nev, -1).astype(np.int64)
+    sur_maps = sur_d.cpu().numpy().astype(np.int64)
+    mc_maps = mc.maps.cpu().numpy().reshape(nev, -1).astype(np.int64)
     tot_mc = mc_maps.sum(0).astype(np.float64)
     tot_sur = sur_maps.sum(0).astype(np.float64)
     rel = (tot_sur - tot_mc) / np.maximum(tot_mc, 1)
     out = {"events": nev, "deposits": int(dep.xyzt.shape[0]), "photons_mc": int(mc.n_photons),
+           "io": "device-resident deposit tables and maps on both paths",
            "fallback_deposit_fraction": float(1 - use.mean()),
            "fallback_photon_fraction": float(dep.nph[~use].sum() / max(dep.nph.sum(), 1)),
            "mc_s": t_mc, "surrogate_s": t_sur, "surrogate_sampling_only_s": t_sur_only,

@@ -271,3 +271,18 @@ def test_events_multibox_matches_per_deposit_oracle(ctx):
     assert np.array_equal(res.maps.reshape(12, -1), maps)
     assert np.array_equal(res.npe, maps.sum(1))
     assert np.array_equal(res.tmin.reshape(12, -1), tmin)
+    # the same queue from device-resident tables into device-resident maps
+    import torch
+    dev = torch.device("cuda", 0)
+    darr = tuple(torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                 for a in (dep.xyzt, dep.nph, dep.event, dep_box))
+    out = (torch.zeros((12, 8, 8), dtype=torch.int32, device=dev),
+           torch.zeros((12, 8, 8), dtype=torch.float32, device=dev),
+           torch.zeros(12, dtype=torch.int32, device=dev))
+    for rep in range(2):
+        r2 = E.transport_events_multibox(boxes, None, K.C2_OP, RO8, dep, seed=9, tau_ns=K.C2_TAU,
+                                         pid0=pid0, ctx=ctx, device_arrays=darr, out=out)
+        assert r2.n_photons == int(nph.sum())
+        assert np.array_equal(out[0].cpu().numpy().reshape(12, -1), maps), rep
+        assert np.array_equal(out[1].cpu().numpy().reshape(12, -1), tmin), rep
+        assert np.array_equal(out[2].cpu().numpy(), maps.sum(1)), rep
