@@ -172,6 +172,37 @@ def test_events_generic_and_specialized_optics(ctx):
         assert np.array_equal(res.npe, ref["npe"]), k
 
 
+def test_events_beyond_2_31_photons_in_one_launch(ctx):
+    """One launch past 2^31 photons (64-bit photon numbers, offsets and queue cursor), on the
+    CsI(Tl)/Teflon optics, checked by size-independent properties: the photon total, npe
+    equal to every map's sum, two launches identical, and the detected fraction equal within
+    5 sigma to the oracle's on the same deposits at 200 photons each."""
+    rng = np.random.default_rng(5)
+    nd = 1000
+    L = np.array(K.C1_BOX.L, np.float32)
+    xyz = (rng.uniform(0.05, 0.95, (nd, 3)) * L).astype(np.float32)
+    xyzt = np.column_stack([xyz, np.zeros(nd)]).astype(np.float32)
+    nph = np.full(nd, 2_150_000, np.int32)
+    ev = np.arange(nd, dtype=np.int32) // 4
+    dep = E.Deposits(xyzt, nph, ev, np.zeros(nd, np.float32), nd // 4)
+    total = int(nph.astype(np.int64).sum())
+    assert total > 2 ** 31
+    a = E.transport_events(K.C1_BOX, K.C1_OP, RO8, dep, seed=3, tau_ns=1000.0, ctx=ctx)
+    assert a.n_photons == total
+    maps = a.maps.reshape(dep.n_events, -1).astype(np.int64)
+    assert np.array_equal(a.npe.astype(np.int64), maps.sum(1))
+    b = E.transport_events(K.C1_BOX, K.C1_OP, RO8, dep, seed=3, tau_ns=1000.0, ctx=ctx)
+    assert np.array_equal(a.maps, b.maps) and np.array_equal(a.tmin, b.tmin)
+    small = E.Deposits(xyzt, np.full(nd, 200, np.int32), ev, np.zeros(nd, np.float32), nd // 4)
+    ref = oracle.run_events(K.C1_BOX, K.C1_OP, RO8, small.xyzt, small.nph, small.event,
+                            small.n_events, seed=3, tau_ns=1000.0)
+    n_small = 200 * nd
+    p_ref = ref["npe"].sum() / n_small
+    p_big = maps.sum() / total
+    sigma = np.sqrt(p_ref * (1 - p_ref) / n_small)
+    assert abs(p_big - p_ref) < 5 * sigma, (p_big, p_ref, sigma)
+
+
 def test_events_schedule_independent_and_autotune(ctx, bgo_events):
     dep, _ = bgo_events
     ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
