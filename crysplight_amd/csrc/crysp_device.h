@@ -6,7 +6,10 @@
 // transcendentals (log, sin, cos) are evaluated in binary64 and rounded once to
 // binary32; Julia's Float32 versions are computed the same way and are correctly
 // rounded in practice (explicit fma() calls below are part of those binary64 kernels,
-// not contractions of the Float32 physics).
+// not contractions of the Float32 physics).  Two comparisons are first tried on hardware
+// estimates (v_log_f32, v_sqrt_f32, v_rcp_f32) whose error bounds are pinned by tests; a
+// lane whose estimate lies within a band of the threshold computes the exact value, so the
+// outcome of every comparison is that of the exact arithmetic.
 #pragma once
 
 #include <hip/hip_runtime.h>
