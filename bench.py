@@ -347,7 +347,12 @@ def _cpu_baseline(box, op, ro, dep, tau, args, maps_gpu, npe_gpu):
     import numpy as np
     import oracle
 
-    cores = max(1, min(16, os.cpu_count() or 1))
+    # every core this process may run on (the affinity mask, not the machine's count),
+    # within the OpenMP thread budget the host grants when it sets one
+    cores = max(1, len(os.sched_getaffinity(0)))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
     cum = np.cumsum(dep.nph.astype(np.int64))
 
     def sample(n_ev):

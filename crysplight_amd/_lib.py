@@ -175,3 +175,54 @@ def ptr(a) -> int | None:
 
 def env_flag(name: str) -> bool:
     return os.environ.get(name, "") not in ("", "0")
+
+
+def _is_torch(a) -> bool:
+    return hasattr(a, "data_ptr") and hasattr(a, "is_contiguous")
+
+
+def check_array(a, name: str, dtype, shape=None, *, numel=None, device: int | None = None):
+    """Validate an array that crosses the C-ABI as a raw pointer: dtype, C-contiguity,
+    shape (or element count) and, for a torch tensor on the GPU, its device.  The library
+    trusts the sizes it is given, so a wrong dtype or a short buffer would otherwise turn
+    into out-of-bounds device reads or writes.  Raises ValueError."""
+    import numpy as np
+    if a is None:
+        raise ValueError(f"{name}: missing array")
+    want = np.dtype(dtype)
+    if _is_torch(a):
+        import torch
+        got = torch.empty(0, dtype=a.dtype).numpy().dtype if a.dtype != torch.bfloat16 else None
+        if got != want:
+            raise ValueError(f"{name}: dtype {a.dtype}, expected {want}")
+        if not a.is_contiguous():
+            raise ValueError(f"{name}: tensor is not contiguous")
+        if a.device.type == "cuda" and device is not None and a.device.index != device:
+            raise ValueError(f"{name}: tensor on cuda:{a.device.index}, the context is on "
+                             f"device {device}")
+        shp, n = tuple(a.shape), a.numel()
+    else:
+        if not isinstance(a, np.ndarray):
+            raise ValueError(f"{name}: expected a numpy array or a torch tensor")
+        if a.dtype != want:
+            raise ValueError(f"{name}: dtype {a.dtype}, expected {want}")
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"{name}: array is not C-contiguous")
+        shp, n = a.shape, a.size
+    if shape is not None and tuple(shp) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(shp)}, expected {tuple(shape)}")
+    if numel is not None and n != numel:
+        raise ValueError(f"{name}: {n} elements, expected {numel}")
+
+
+def ready(*arrays) -> None:
+    """The library works on its own HIP stream.  Device tensors handed to it must hold
+    their final contents when the call starts, so work that torch still has queued for
+    them (a fill, a copy, the kernel that produced them) is waited for first."""
+    import sys
+    if "torch" not in sys.modules:
+        return
+    import torch
+    devs = {a.device for a in arrays if _is_torch(a) and a.device.type == "cuda"}
+    for d in devs:
+        torch.cuda.current_stream(d).synchronize()

@@ -59,6 +59,7 @@ struct KParams {
     int32_t chunk;                            // photons per work-queue grab
     int32_t refill_min;                       // refill lanes once this many are idle
     int32_t hist_words;                       // nx*ny*(nbins+1)
+    int32_t nbox;                             // crystals of a several-crystal event queue
 };
 
 template <uint32_t OPT>

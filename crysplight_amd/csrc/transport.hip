@@ -173,7 +173,10 @@ transport_kernel(const KParams P, const KBuffers B)
                     ev = B.dep_event[d];
                     t_start = t0;
                     if (MULTIBOX) {
-                        const float4 bx = B.boxes[B.dep_box[d]];
+                        // a crystal index outside the table reads crystal 0 (the check
+                        // kernel has flagged the call as failed)
+                        const int32_t bi = B.dep_box[d];
+                        const float4 bx = B.boxes[(uint32_t)bi < (uint32_t)P.nbox ? bi : 0];
                         bL0 = bx.x; bL1 = bx.y; bL2 = bx.z;
                     }
                 } else {
@@ -284,10 +287,9 @@ __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *
 
 // exclusive scan of the per-deposit photon counts (device-resident inputs); one
 // workgroup, each thread a contiguous slice.  off[ndep] = total.  flag bit 1: a negative
-// count (taken as 0), bit 2: an event index outside [0, nev).
-__global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, const int32_t *ev,
-                                                      int64_t ndep, int64_t nev, int64_t *off,
-                                                      unsigned int *flag)
+// count (taken as 0).
+__global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, int64_t ndep,
+                                                      int64_t *off, unsigned int *flag)
 {
     __shared__ int64_t part[1024];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -300,8 +302,6 @@ __global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, const 
         const int32_t v = nph[i];
         if (v < 0) bad |= 1u;
         s += v > 0 ? v : 0;
-        const int32_t e = ev[i];
-        if (e < 0 || (int64_t)e >= nev) bad |= 2u;
     }
     if (bad) atomicOr(flag, bad);
     part[tid] = s;
@@ -314,6 +314,26 @@ __global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, const 
     __syncthreads();
     int64_t acc = part[tid];
     for (int64_t i = lo; i < hi; ++i) { off[i] = acc; const int32_t v = nph[i]; acc += v > 0 ? v : 0; }
+}
+
+// Index checks of a deposit table on the device, for tables that live there: flag bit 2
+// an event index outside [0, nev), bit 4 a crystal index outside [0, nbox) (dep_box may be
+// null).  The transport kernel guards both reads itself; these bits turn a bad table into
+// an error for the caller.
+__global__ void check_deposits(const int32_t *ev, const int32_t *box, int64_t ndep, int64_t nev,
+                               int32_t nbox, unsigned int *flag)
+{
+    unsigned bad = 0u;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ndep;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t e = ev[i];
+        if (e < 0 || (int64_t)e >= nev) bad |= 2u;
+        if (box) {
+            const int32_t b = box[i];
+            if (b < 0 || b >= nbox) bad |= 4u;
+        }
+    }
+    if (bad) atomicOr(flag, bad);
 }
 
 __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids, int n_draws,
@@ -411,6 +431,7 @@ struct crysp_ctx {
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_key;
 };
+
 
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
@@ -865,26 +886,29 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
             if (dep_nph[d] < 0) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
             off[d + 1] = off[d] + dep_nph[d];
         }
-        if (!is_device_ptr(dep_event))
-            for (int64_t d = 0; d < ndep; ++d)
-                if (dep_event[d] < 0 || dep_event[d] >= nev)
-                    return fail(CRYSP_EINVAL, "dep_event out of range");
         ntot_host = off[ndep];
         HIPCHK(hipMemcpyAsync(doff, off.data(), (size_t)(ndep + 1) * 8, hipMemcpyHostToDevice,
                               c->stream));
     }
+    const bool ev_dev = ndep > 0 && is_device_ptr(dep_event);
+    if (!ev_dev)
+        for (int64_t d = 0; d < ndep; ++d)
+            if (dep_event[d] < 0 || dep_event[d] >= nev)
+                return fail(CRYSP_EINVAL, "dep_event out of range");
     if ((rc = stage_in(c, B_DEP, dep_xyzt, (size_t)ndep * 16, (const void **)&B.dep)) ||
         (rc = stage_in(c, B_DEPEV, dep_event, (size_t)ndep * 4, (const void **)&B.dep_event)))
         return rc;
     const int32_t *nph_d = nph_dev ? dep_nph : nullptr;
-    void *dmaps, *dtmin, *dnpe, *dkeys;
+    void *dmaps, *dtmin, *dnpe;
     const size_t cells = (size_t)nev * nxy;
     if ((rc = out_buf(c, B_MAPS, maps, cells * 4, &dmaps)) ||
         (rc = out_buf(c, B_TMIN, tmin, cells * 4, &dtmin)) ||
-        (rc = out_buf(c, B_NPE, npe, (size_t)nev * 4, &dnpe)) ||
-        (rc = ensure(c, B_EVKEYS, cells * 4, &dkeys)))
+        (rc = out_buf(c, B_NPE, npe, (size_t)nev * 4, &dnpe)))
         return rc;
     B.maps = (int32_t *)dmaps;
+    // first-hit times are gathered as order-preserving keys in the tmin buffer itself and
+    // turned into floats in place by finish_events
+    void *dkeys = dtmin;
     B.ev_tkeys = (uint32_t *)dkeys;
     B.ev_tsum = tsum_dev;
     const bool multibox = boxes != nullptr;
@@ -903,6 +927,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         if (!is_device_ptr(dep_box))
             for (int64_t d = 0; d < ndep; ++d)
                 if (dep_box[d] < 0 || dep_box[d] >= nbox) return fail(CRYSP_EINVAL, "dep_box out of range");
+        P.nbox = nbox;
         void *dbt;
         if ((rc = ensure(c, B_BOXTAB, (size_t)nbox * 16, &dbt))) return rc;
         HIPCHK(hipMemcpyAsync(dbt, boxtab.data(), (size_t)nbox * 16, hipMemcpyHostToDevice, c->stream));
@@ -927,10 +952,12 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
 
     // everything the captured nodes read: parameters, every buffer address (the kernel's
     // and the outputs finish_events writes), the launch shape and the variant
-    std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 6 * sizeof(int64_t));
+    const bool box_dev = multibox && is_device_ptr(dep_box);
+    std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 8 * sizeof(int64_t));
     {
-        int64_t extra[6] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
-                             (int64_t)c->timing, (int64_t)nph_dev };
+        int64_t extra[8] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
+                             (int64_t)c->timing, (int64_t)nph_dev, (int64_t)ev_dev,
+                             (int64_t)box_dev };
         const void *ptrs[3] = { nph_d, dtmin, dnpe };
         unsigned char *q = key.data();
         std::memcpy(q, &P, sizeof P); q += sizeof P;
@@ -944,8 +971,15 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
         hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)dflag, 0u, 4);
         if (nph_d)
-            hipLaunchKernelGGL(scan_deposits, dim3(1), dim3(1024), 0, c->stream, nph_d,
-                               B.dep_event, ndep, nev, (int64_t *)doff, (unsigned int *)dflag);
+            hipLaunchKernelGGL(scan_deposits, dim3(1), dim3(1024), 0, c->stream, nph_d, ndep,
+                               (int64_t *)doff, (unsigned int *)dflag);
+        // device-resident index tables are checked on the device (host ones were checked
+        // before any launch)
+        const int32_t *box_chk = box_dev ? B.dep_box : nullptr;
+        if (ndep > 0 && (ev_dev || box_chk))
+            hipLaunchKernelGGL(check_deposits, dim3((unsigned)std::min<int64_t>(1024, (ndep + 255) / 256)),
+                               dim3(256), 0, c->stream, B.dep_event, box_chk, ndep, nev, nbox,
+                               (unsigned int *)dflag);
         hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
                            (int64_t)cells);
         hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
@@ -1015,7 +1049,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     c->last_n_hint = tail[0];
     if (n_out) *n_out = tail[0];
     if (tail[1] & 1) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
-    if (tail[1] & 2) return fail(CRYSP_EINVAL, "dep_event out of range (those deposits were skipped)");
+    if (tail[1] & 2) return fail(CRYSP_EINVAL, "dep_event out of range (those deposits' detections were dropped)");
+    if (tail[1] & 4) return fail(CRYSP_EINVAL, "dep_box out of range (those deposits were traced in crystal 0)");
     if (ndep > 0) return finish_timing(c);
     return CRYSP_OK;
 }

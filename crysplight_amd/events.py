@@ -178,6 +178,22 @@ def rand_poisson(lam: float, n: int, seed: int = 1) -> np.ndarray:
     return out
 
 
+def _check_event_arrays(ctx, ndep: int, nev: int, nxy: int, inputs, outputs) -> None:
+    """Types and sizes of an event call's arrays (numpy or torch, host or device), then
+    wait for torch's queued work on the device ones (the library runs on its own stream)."""
+    xyzt, nph, ev = inputs[:3]
+    _lib.check_array(xyzt, "xyzt", np.float32, (ndep, 4), device=ctx.device)
+    _lib.check_array(nph, "nph", np.int32, (ndep,), device=ctx.device)
+    _lib.check_array(ev, "event", np.int32, (ndep,), device=ctx.device)
+    if len(inputs) > 3:
+        _lib.check_array(inputs[3], "dep_box", np.int32, (ndep,), device=ctx.device)
+    maps, tmin, npe = outputs
+    _lib.check_array(maps, "maps", np.int32, numel=nev * nxy, device=ctx.device)
+    _lib.check_array(tmin, "tmin", np.float32, numel=nev * nxy, device=ctx.device)
+    _lib.check_array(npe, "npe", np.int32, numel=nev, device=ctx.device)
+    _lib.ready(*inputs, *outputs)
+
+
 @dataclass
 class EventMaps:
     maps: np.ndarray       # (n_events, ny, nx) int32 view of the (nx, ny, n) column-major array
@@ -212,6 +228,7 @@ def transport_events(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: in
     em = Emission(tau_ns, tau_rise_ns)
     ntot = C.c_int64()
     ndep = int(dep.xyzt.shape[0]) if device_arrays is None else int(xyzt.shape[0])
+    _check_event_arrays(ctx, ndep, nev, nxy, (xyzt, nph, ev), (maps, tmin, npe))
     rc = _lib.load().crysp_run_events(
         ctx.handle, C.byref(box), C.byref(op), C.byref(ro), _lib.ptr(xyzt), _lib.ptr(nph),
         _lib.ptr(ev), ndep, nev, int(seed), int(pid0), int(max_bounces), C.byref(em),
@@ -293,6 +310,8 @@ def transport_events_multibox(boxes, dep_box, op: OpticalParams, ro, dep: Deposi
         npe = np.zeros(nev, np.int32)
     em = Emission(tau_ns, tau_rise_ns)
     ntot = C.c_int64()
+    _check_event_arrays(ctx, ndep, nev, ro.grid.nx * ro.grid.ny, (xyzt, nph, ev, db),
+                        (maps, tmin, npe))
     lib = _lib.load()
     f = lib.crysp_run_events_multibox
     f.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(OpticalParams),

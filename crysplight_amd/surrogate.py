@@ -131,6 +131,9 @@ class MLP:
             out = np.zeros((n, self.n_out), np.float32)
             _mcheck(_bind().crysp_mlp_forward(self._h, Xh.ctypes.data, n, out.ctypes.data), "forward")
             return out
+        _lib.check_array(X, "X", np.float32, (n, self.n_in))
+        _lib.check_array(out, "out", np.float32, (n, self.n_out))
+        _lib.ready(X, out)
         _mcheck(_bind().crysp_mlp_forward(self._h, _ptr(X), n, _ptr(out)), "forward")
         return out
 
@@ -149,6 +152,15 @@ class MLP:
                       weight_decay=0.0, mean_row_weight=1.0):
         """Adam steps over consecutive rows of device tensors X, T, W."""
         losses = np.zeros(max(1, n_batches), np.float32)
+        rows = int(n_batches) * int(batch)
+        for a, name, cols in ((X, "X", self.n_in), (T, "T", self.n_out), (W, "W", self.n_out)):
+            if a is None and name == "W":
+                continue
+            _lib.check_array(a, name, np.float32)
+            if a.ndim != 2 or a.shape[1] != cols or a.shape[0] < rows:
+                raise ValueError(f"{name}: shape {tuple(a.shape)}, expected at least "
+                                 f"({rows}, {cols})")
+        _lib.ready(*(a for a in (X, T, W) if a is not None))
         _mcheck(_bind().crysp_mlp_train_batches(self._h, _ptr(X), _ptr(T), _ptr(W), int(n_batches),
                                                 int(batch), lr, betas[0], betas[1], eps,
                                                 weight_decay, mean_row_weight,
@@ -449,6 +461,15 @@ def sample_events(grid, box: Box, dep: Deposits, ev_first, use, yield_per_mev: f
     else:
         maps, tmean = out
     use8 = np.ascontiguousarray(use, np.uint8) if isinstance(use, np.ndarray) else use
+    ndep = int(dep.xyzt.shape[0])
+    _lib.check_array(grid, "grid", np.float32, (gz, gy, gx, 2, ncell), device=device)
+    _lib.check_array(dep.xyzt, "xyzt", np.float32, (ndep, 4), device=device)
+    _lib.check_array(dep.e_kev, "e_kev", np.float32, (ndep,), device=device)
+    _lib.check_array(use8, "use", np.uint8, (ndep,), device=device)
+    _lib.check_array(ev_first, "ev_first", np.int64, (nev + 1,), device=device)
+    _lib.check_array(maps, "maps", np.int32, numel=nev * ncell, device=device)
+    _lib.check_array(tmean, "tmean", np.float32, numel=nev * ncell, device=device)
+    _lib.ready(grid, dep.xyzt, dep.e_kev, use8, ev_first, maps, tmean)
     rc = lib.crysp_surrogate_sample_events(
         device, _ptr(grid), gx, gy, gz, ncell, C.byref(box), _ptr(dep.xyzt), _ptr(dep.e_kev),
         _ptr(use8), _ptr(ev_first), int(dep.xyzt.shape[0]), nev, float(yield_per_mev),

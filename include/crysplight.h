@@ -10,6 +10,12 @@
  *    pinned; the library stages it through its own device buffers) or device memory
  *    from hipMalloc / a torch tensor on the context's device (used in place).  All
  *    calls are synchronous, as the reference's KA.synchronize (src/kernel.jl:335).
+ *  - Ordering: the library runs its kernels on a stream of its own and does not wait for
+ *    work the caller queued on other streams.  Device arrays must hold their final
+ *    contents when a call starts: wait for their producers first (for torch tensors,
+ *    synchronize torch's current stream).  When a call returns, its outputs are complete.
+ *  - Device-resident index tables (event and crystal indices) are checked on the device;
+ *    an index out of range makes the call return CRYSP_EINVAL after its launch.
  *  - Photon ids follow the reference: photon k (1-based within a call) draws from the
  *    Philox stream (seed, pid = k + pid0) (src/kernel.jl:162-163).
  *  - Multi-dimensional outputs are column-major like the Julia arrays they replace:

@@ -192,3 +192,43 @@ def test_synthetic_deposits_ranges():
     lam = 54.0 * d.e_kev
     z = (d.nph - lam) / np.sqrt(lam)
     assert abs(z.mean()) < 0.05 and abs(z.std() - 1) < 0.05
+
+
+class _NoCtx:
+    """Stands in for a context: the argument checks run before any library call."""
+    device = 0
+    handle = None
+
+
+def test_event_wrappers_reject_bad_arrays_before_calling():
+    dep = E.Deposits(np.zeros((4, 4), np.float32), np.ones(4, np.int32), np.arange(4, dtype=np.int32),
+                     np.ones(4, np.float32), 4)
+    ro = P.readout(P.SipmGrid(3.2, 8, 8))
+    good_in = (dep.xyzt, dep.nph, dep.event)
+    good_out = (np.zeros((4, 8, 8), np.int32), np.zeros((4, 8, 8), np.float32), np.zeros(4, np.int32))
+    bad_cases = [
+        ((dep.xyzt.astype(np.float64), dep.nph, dep.event), good_out, "xyzt"),
+        ((dep.xyzt, dep.nph.astype(np.int64), dep.event), good_out, "nph"),
+        ((dep.xyzt, dep.nph, dep.event[:3]), good_out, "event"),
+        ((np.zeros((4, 8), np.float32)[:, ::2], dep.nph, dep.event), good_out, "xyzt"),
+        (good_in, (np.zeros((3, 8, 8), np.int32), good_out[1], good_out[2]), "maps"),
+        (good_in, (good_out[0], np.zeros((4, 8, 8), np.float64), good_out[2]), "tmin"),
+        (good_in, (good_out[0], good_out[1], np.zeros(5, np.int32)), "npe"),
+    ]
+    for inp, out, what in bad_cases:
+        with pytest.raises(ValueError, match=what):
+            E.transport_events(K.C2_BOX, K.C2_OP, ro, dep, seed=1, tau_ns=300.0, ctx=_NoCtx(),
+                               device_arrays=inp, out=out)
+    with pytest.raises(ValueError, match="dep_box"):
+        E.transport_events_multibox([K.C2_BOX], None, K.C2_OP, ro, dep, seed=1, tau_ns=300.0,
+                                    ctx=_NoCtx(), device_arrays=good_in + (np.zeros(4, np.int16),),
+                                    out=good_out)
+
+
+def test_check_array_torch_rules():
+    torch = pytest.importorskip("torch")
+    _lib.check_array(torch.zeros((2, 4)), "x", np.float32, (2, 4))
+    with pytest.raises(ValueError, match="contiguous"):
+        _lib.check_array(torch.zeros((4, 4))[:, :2], "x", np.float32)
+    with pytest.raises(ValueError, match="dtype"):
+        _lib.check_array(torch.zeros(3, dtype=torch.int64), "n", np.int32)

@@ -317,3 +317,36 @@ def test_events_multibox_matches_per_deposit_oracle(ctx):
         assert np.array_equal(out[0].cpu().numpy().reshape(12, -1), maps), rep
         assert np.array_equal(out[1].cpu().numpy().reshape(12, -1), tmin), rep
         assert np.array_equal(out[2].cpu().numpy(), maps.sum(1)), rep
+
+
+def test_device_tables_with_bad_indices_raise(ctx, bgo_events):
+    """Device-resident event and crystal indices are checked on the device: an index out
+    of range makes the call fail (CRYSP_EINVAL) instead of reading outside the tables, and
+    the context stays usable."""
+    import torch
+    dep, _ = bgo_events
+    dev = torch.device("cuda", 0)
+    nev = dep.n_events
+    out = (torch.zeros((nev, 8, 8), dtype=torch.int32, device=dev),
+           torch.zeros((nev, 8, 8), dtype=torch.float32, device=dev),
+           torch.zeros(nev, dtype=torch.int32, device=dev))
+    ev_bad = dep.event.copy()
+    ev_bad[5] = nev                                   # one past the last event
+    darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, ev_bad))
+    with pytest.raises(CryspError, match="dep_event"):
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=darr, out=out)
+    # host counts with device event indices take the same check
+    darr_h = (torch.from_numpy(dep.xyzt).to(dev), dep.nph.copy(), torch.from_numpy(ev_bad).to(dev))
+    with pytest.raises(CryspError, match="dep_event"):
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=darr_h, out=out)
+    boxes = [K.C2_BOX, Box((25.6, 25.6, 10.0))]
+    box_bad = np.zeros(dep.xyzt.shape[0], np.int32)
+    box_bad[3] = 2                                    # only crystals 0 and 1 exist
+    darr4 = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event, box_bad))
+    with pytest.raises(CryspError, match="dep_box"):
+        E.transport_events_multibox(boxes, None, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU,
+                                    ctx=ctx, device_arrays=darr4, out=out)
+    res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+    _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU), nev)
