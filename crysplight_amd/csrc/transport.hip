@@ -430,8 +430,41 @@ struct crysp_ctx {
     // captured event-mode graph
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_key;
+    // the device's copy of the file-scope scratch array (allocation-free entry point)
+    void *scratch = nullptr;
 };
 
+// Scratch memory of crysp_run_events_noalloc: a file-scope device array, so that the
+// entry point needs no allocation.  Each device holds its own copy for the life of the
+// process; every call on a device uses that device's copy, so calls that overlap on one
+// device race.  Layout: the work-queue cursor, the error flags, the deposits' photon
+// offsets (NA_MAX_DEPOSITS + 1 int64).
+constexpr int64_t NA_MAX_DEPOSITS = (int64_t)1 << 20;
+constexpr size_t NA_CURSOR = 0, NA_FLAG = 64, NA_DEPOFF = 256;
+constexpr size_t NA_BYTES = NA_DEPOFF + 8 * (size_t)(NA_MAX_DEPOSITS + 1);
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// The scratch array's address on `device` (current), looked up once per device.  The
+// array's host-side name is a stand-in, not a device address: only this pointer is
+// passed to kernels and copies.
+static void *scratch_address(int device)
+{
+    static std::mutex mu;
+    static void *addr[64] = {};
+    static bool done[64] = {};
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+        }
+        addr[device] = p;
+        done[device] = p != nullptr;
+    }
+    return addr[device];
+}
 
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
@@ -658,6 +691,7 @@ crysp_ctx *crysp_create(int device)
     c->n_cu = prop.multiProcessorCount;
     c->max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
     c->lds_per_block = prop.sharedMemPerBlock;
+    c->scratch = scratch_address(device);           // device current (hipSetDevice above)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -855,7 +889,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                        uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
                        float *tmin, int32_t *npe, int64_t *n_out, bool use_graph,
                        unsigned long long *tsum_dev = nullptr, const crysp_box *boxes = nullptr,
-                       int32_t nbox = 0, const int32_t *dep_box = nullptr)
+                       int32_t nbox = 0, const int32_t *dep_box = nullptr,
+                       void *noalloc = nullptr)
 {
     if (!c) return fail(CRYSP_EINVAL, "null context");
     if (ndep < 0 || nev < 0) return fail(CRYSP_EINVAL, "negative sizes");
@@ -863,6 +898,19 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         return fail(CRYSP_EINVAL, "null deposit arrays");
     if (nev > 0 && (!maps || !tmin || !npe)) return fail(CRYSP_EINVAL, "null outputs");
     if (nev > INT32_MAX) return fail(CRYSP_ERANGE, "more than 2^31-1 events");
+    if (noalloc) {
+        // everything in device memory already, nothing staged, no graph, no counters
+        if (ndep > NA_MAX_DEPOSITS)
+            return fail(CRYSP_ERANGE, "more than " + std::to_string(NA_MAX_DEPOSITS) +
+                                      " deposits for the allocation-free entry point");
+        if ((ndep > 0 && (!is_device_ptr(dep_xyzt) || !is_device_ptr(dep_nph) ||
+                          !is_device_ptr(dep_event))) ||
+            (nev > 0 && (!is_device_ptr(maps) || !is_device_ptr(tmin) || !is_device_ptr(npe))))
+            return fail(CRYSP_EINVAL, "the allocation-free entry point takes device arrays only");
+        if (c->count_ops)
+            return fail(CRYSP_EINVAL, "operation counting needs the allocating entry point");
+        if (boxes || tsum_dev) return fail(CRYSP_EINVAL, "single crystal, no time sums");
+    }
     KParams P;
     int rc = fill_params(P, box, op, ro, em, seed, pid0, maxb);
     if (rc) return rc;
@@ -873,9 +921,13 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     P.nev = nev;
     KBuffers B{};
     void *doff, *dflag;
-    if ((rc = ensure(c, B_DEPOFF, (size_t)(ndep + 1) * 8, &doff)) ||
-        (rc = ensure(c, B_FLAG, 16, &dflag)))
+    if (noalloc) {
+        doff = (char *)noalloc + NA_DEPOFF;
+        dflag = (char *)noalloc + NA_FLAG;
+    } else if ((rc = ensure(c, B_DEPOFF, (size_t)(ndep + 1) * 8, &doff)) ||
+               (rc = ensure(c, B_FLAG, 16, &dflag))) {
         return rc;
+    }
     B.dep_off = (const int64_t *)doff;
     B.flag = (unsigned int *)dflag;
     int64_t ntot_host = -1;
@@ -935,7 +987,8 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         if ((rc = stage_in(c, B_DEPBOX, dep_box, (size_t)ndep * 4, (const void **)&B.dep_box))) return rc;
     }
     void *cur;
-    if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
+    if (noalloc) cur = (char *)noalloc + NA_CURSOR;
+    else if ((rc = ensure(c, B_CURSOR, 8, &cur))) return rc;
     B.cursor = (unsigned long long *)cur;
     if (c->count_ops) {
         void *ops;
@@ -1007,7 +1060,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
         return CRYSP_OK;
     };
     bool launched = false;
-    if (use_graph) {
+    if (use_graph && !noalloc) {
         if (c->graph && key == c->graph_key) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             HIPCHK(hipGraphLaunch(c->graph, c->stream));
@@ -1063,6 +1116,19 @@ int crysp_run_events(crysp_ctx *c, const crysp_box *box, const crysp_optical_par
 {
     return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
                        maxb, em, maps, tmin, npe, n_out, true);
+}
+
+int crysp_run_events_noalloc(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                             const crysp_readout *ro, const float *dep_xyzt, const int32_t *dep_nph,
+                             const int32_t *dep_event, int64_t ndep, int64_t nev, uint64_t seed,
+                             uint64_t pid0, int32_t maxb, const crysp_emission *em, int32_t *maps,
+                             float *tmin, int32_t *npe, int64_t *n_out)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (!c->scratch) return fail(CRYSP_EHIP, "no address for the device scratch array");
+    return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, pid0,
+                       maxb, em, maps, tmin, npe, n_out, false, nullptr, nullptr, 0, nullptr,
+                       c->scratch);
 }
 
 int crysp_run_events_multibox(crysp_ctx *c, const crysp_box *boxes, int32_t nbox,

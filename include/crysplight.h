@@ -153,6 +153,22 @@ int crysp_run_events(crysp_ctx *ctx, const crysp_box *box,
                      const crysp_emission *em,
                      int32_t *maps, float *tmin, int32_t *npe, int64_t *n_photons_out);
 
+/* crysp_run_events without any allocation: no reference counterpart.  It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every array must already be device memory on the
+ * context's device; no graph is captured and operation counting must be off.  Scratch
+ * memory (queue cursor, error flags, deposit photon offsets) is a file-scope device array
+ * of about 8 MiB, held by each device that runs the library until the process ends:
+ * ndep <= 2^20 (CRYSP_ERANGE beyond).  All calls on one device share that array, so calls
+ * that overlap on one device race.  Results equal crysp_run_events'. */
+int crysp_run_events_noalloc(crysp_ctx *ctx, const crysp_box *box,
+                             const crysp_optical_params *op, const crysp_readout *ro,
+                             const float *dep_xyzt, const int32_t *dep_nph,
+                             const int32_t *dep_event, int64_t ndep, int64_t nev,
+                             uint64_t seed, uint64_t pid0, int32_t max_bounces,
+                             const crysp_emission *em, int32_t *maps, float *tmin,
+                             int32_t *npe, int64_t *n_photons_out);
+
 /* Event mode with deposits in several crystals sharing one queue (no reference
  * counterpart): deposit d lies in crystal boxes[dep_box[d]] (nbox <= 64); everything
  * else as crysp_run_events. */
