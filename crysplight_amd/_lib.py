@@ -40,6 +40,14 @@ SIGNATURES = {
     "crysp_set_queue": (C.c_int, [_vp, _i32, _i32]),
     "crysp_autotune_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
                                         _i32, _pem, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
+    "crysp_autotune_events_queue": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64,
+                                              _u64, _i32, _pem, _vp, _i32, _vp, _i32, _vp, _i32,
+                                              _vp, _i32, _i32, _vp, _vp]),
+    "crysp_autotune_accumulate": (C.c_int, [_vp, _pbox, _pop, _pro, _ptb, _vp, _vp, _vp, _vp,
+                                            _i32, _i64, _u64, _i32, _pem, _vp, _i32, _vp, _i32,
+                                            _vp, _i32, _vp, _i32, _i32, _vp, _vp]),
+    "crysp_run_events_noalloc": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64,
+                                           _u64, _u64, _i32, _pem, _vp, _vp, _vp, _vp]),
     "crysp_debug_draws": (C.c_int, [_vp, _u64, _u64, _i64, _i32, _vp]),
     "crysp_debug_math": (C.c_int, [_vp, _i32, _vp, _i64, _vp]),
     "crysp_set_timing": (C.c_int, [_vp, _i32]),

@@ -208,8 +208,10 @@ struct RowArgs {
     float *bpart[MAXL];                             // [n_wg][out] bias-gradient partials
     float *lpart;                                   // [n_wg] loss partials
     const float *Xraw, *T, *W, *shift, *scale;
+    const float *R0;                                // rate offsets [M][n_pois] or null
     float inv_wsum;
     const int64_t *bidx;                            // batch index (graph replays) or null
+    float *Y;                                       // inference: the outputs [M][n_out] fp32
 };
 
 // The weights of a workgroup's GEMMs, in the order mlp_rows runs them (forward layers
@@ -223,14 +225,16 @@ struct BStream {
     int Q;                                          // phases in all
     bf16x8 buf[CPT];
 
+
     __device__ __forceinline__ void gemm(int g, const __bf16 *&B, int &ldb, int &N, int &K) const
     {
         if (g < a.L) { B = a.Wt[g]; ldb = a.dpad[g]; N = a.dout[g]; K = a.dpad[g]; }
         else { const int l = 2 * a.L - 1 - g; B = a.Wn[l]; ldb = a.dout[l]; N = a.din[l]; K = a.dout[l]; }
     }
-    __device__ __forceinline__ BStream(const RowArgs &ra) : a(ra), Q(0)
+    // the forward GEMMs only (inference), or forward and backward (training step)
+    __device__ __forceinline__ BStream(const RowArgs &ra, bool train) : a(ra), Q(0)
     {
-        for (int g = 0; g < 2 * a.L - 1; ++g) {
+        for (int g = 0; g < (train ? 2 * a.L - 1 : a.L); ++g) {
             const __bf16 *B; int ldb, N, K;
             gemm(g, B, ldb, N, K);
             Q += (K + KB - 1) / KB;
@@ -275,9 +279,9 @@ struct BStream {
 
 // out[RB x N] = A[RB x K] * B^T for the next GEMM of the stream (phases q .. q + np - 1),
 // A resident in LDS.  Wave w owns the 32-column block w, lane element i (0..15) of it at
-// (row, col).  pre(i, row, col, u, v) loads what the epilogue of an element needs into u, v
-// before the K loop, so those loads complete under the MFMAs; epi(i, row, col, acc, u, v)
-// stores the element.
+// (row, col).  pre(i, row, col, u, v, r) loads what the epilogue of an element needs into
+// u, v, r before the K loop, so those loads complete under the MFMAs;
+// epi(i, row, col, acc, u, v, r) stores the element.
 template <class Pre, class Epi>
 __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
                                           BStream &bs, int &q, __bf16 (*sB)[KB + LDS_PAD],
@@ -286,10 +290,11 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const bool mine = wave < N / 32;
     const int col = wave * 32 + (lane & 31);
-    float pu[16], pv[16];
+    float pu[16], pv[16], pr[16];
     if (mine) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) pre(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i]);
+        for (int i = 0; i < 16; ++i)
+            pre(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i], pr[i]);
     }
     f32x16 acc;
 #pragma unroll
@@ -312,7 +317,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     if (mine) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            epi(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i]);
+            epi(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i], pr[i]);
     }
 }
 
@@ -351,7 +356,9 @@ __device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD],
 //   backward data path: dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1});
 //   bias-gradient partials (column sums of every dZ_l over the RB rows) and the loss partial.
 // Padded rows (>= M) carry zero inputs and a zero gradient.
-template <int L>
+// TRAIN = false is the inference pass: the forward layers only, the last one's output (with
+// its bias) stored to a.Y, nothing else written.
+template <int L, bool TRAIN = true>
 __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 {
     __shared__ __bf16 act[2][RB][AMAX + LDS_PAD];
@@ -363,7 +370,8 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
     const float *Xraw = a.Xraw + b * (int64_t)a.M * a.n_in;
     const float *T = a.T + b * (int64_t)a.M * a.n_out;
     const float *W = a.W ? a.W + b * (int64_t)a.M * a.n_out : nullptr;
-    BStream bs(a);
+    const float *R0 = a.R0 ? a.R0 + b * (int64_t)a.M * a.n_pois : nullptr;
+    BStream bs(a, TRAIN);
     bs.load(0);
     int q = 0;
 
@@ -374,7 +382,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         act[0][r][c] = to_bf16(v);
     }
     __syncthreads();
-    rows_store_T(act[0], a.kin_pad, a.XT, a.Mpad, r0);
+    if (TRAIN) rows_store_T(act[0], a.kin_pad, a.XT, a.Mpad, r0);
 
     int cur = 0;
     float lsum = 0.0f;
@@ -388,31 +396,49 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         const float *bias = a.bias[l];
         if (l + 1 < L) {
             rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int row, int col, float &u, float &) { u = bias[col]; },
-                      [&](int i, int row, int col, float v, float u, float) {
+                      [&](int, int row, int col, float &u, float &, float &) { u = bias[col]; },
+                      [&](int i, int row, int col, float v, float u, float, float) {
                           v += u;
-                          gpr[l < L - 1 ? l : 0][i] = gelu_grad(v);
+                          if (TRAIN) gpr[l < L - 1 ? l : 0][i] = gelu_grad(v);
                           out[row][col] = to_bf16(gelu(v));
+                      });
+        } else if (!TRAIN) {
+            const int M = a.M;
+            float *Y = a.Y;
+            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
+                      [&](int, int row, int col, float &u, float &, float &) { u = bias[col]; },
+                      [&](int, int row, int col, float y, float u, float, float) {
+                          const int rg = r0 + row;
+                          if (rg < M) Y[(size_t)rg * N + col] = y + u;
                       });
         } else {
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
             rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int row, int col, float &t, float &w) {
+                      [&](int, int row, int col, float &t, float &w, float &r) {
                           const int rg = r0 + row;
                           const size_t o = (size_t)rg * N + col;
                           t = rg < M ? T[o] : 0.0f;
                           w = rg < M ? (W ? W[o] : 1.0f) : 0.0f;
+                          r = (R0 && rg < M && col < n_pois) ? R0[(size_t)rg * n_pois + col] : 0.0f;
                       },
-                      [&](int, int row, int col, float y, float t, float w) {
+                      [&](int, int row, int col, float y, float t, float w, float r) {
                           y += bias[col];
                           const int rg = r0 + row;
                           float g = 0.0f;
                           if (rg < M) {
                               if (col < n_pois) {
+                                  // rate = e^y + r (r: a known part of the rate, 0 without
+                                  // offsets): w (rate - t log rate), gradient w (1 - t/rate) e^y
                                   const float e = __expf(y);
-                                  lsum += w * (e - t * y) * inv_wsum;
-                                  g = w * (e - t) * inv_wsum;
+                                  if (R0) {
+                                      const float rate = e + r;
+                                      lsum += w * (rate - t * __logf(rate)) * inv_wsum;
+                                      g = w * (e - t * (e / rate)) * inv_wsum;
+                                  } else {
+                                      lsum += w * (e - t * y) * inv_wsum;
+                                      g = w * (e - t) * inv_wsum;
+                                  }
                               } else {
                                   const float d = y - t;
                                   lsum += w * d * d * inv_wsum;
@@ -424,8 +450,9 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         }
         __syncthreads();
         cur ^= 1;
-        rows_store_T(act[cur], N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
+        if (TRAIN) rows_store_T(act[cur], N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
     }
+    if (!TRAIN) return;
     rows_colsum(act[cur], a.dout[L - 1], a.bpart[L - 1] + (size_t)blockIdx.x * a.dout[L - 1]);
 
 #pragma unroll
@@ -433,8 +460,8 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         const int N = a.din[l];                     // = dout[l - 1]
         __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
         rows_gemm(act[cur], a.dout[l], N, bs, q, sB,
-                  [&](int, int, int, float &, float &) {},
-                  [&](int i, int row, int col, float v, float, float) {
+                  [&](int, int, int, float &, float &, float &) {},
+                  [&](int i, int row, int col, float v, float, float, float) {
                       out[row][col] = to_bf16(v * gpr[l - 1][i]);
                   });
         __syncthreads();
@@ -681,6 +708,308 @@ __global__ void fill_f32(float *p, float v, int64_t n)
         p[i] = v;
 }
 
+
+// ------------------------------------------------------------------ direct light ----
+// The part of a point source's light map that reaches the readout plane z = Lz straight
+// from the emission point: a photon emitted at depth d = Lz - z with polar angle theta
+// (u = cos theta) towards the plane lands at distance rho = d tan theta from the foot
+// point (x, y), crosses the crystal -> coupling interface with the Fresnel transmission
+// T(u) (zero beyond total internal reflection), survives the bulk over the path d / u, and
+// is detected with the PDE in the SiPM cell under the landing point.  A landing point
+// outside the face means the photon meets a side wall first: not direct.  For cell c,
+//   D_c = PDE / (4 pi) * int dphi [ H(u(rho_in(phi)), k) - H(u(rho_out(phi)), k) ],
+//   H(u, k) = int_u^1 T(v) e^{-k / v} dv,   k = d / Lambda  (Lambda: absorption and
+//   Rayleigh lengths combined), u(rho) = d / sqrt(rho^2 + d^2),
+// where [rho_in, rho_out] is where the ray at azimuth phi lies in the cell.  H is
+// tabulated on the host; the azimuth integral runs piecewise between the directions of
+// the cell's corners (where rho_in, rho_out change edge) with 8-point Gauss-Legendre.
+struct DirectTable {
+    const float *H;          // [nk][nu]: H at u = uc + (1 - uc) s^2, s = j / (nu - 1)
+    int nu, nk;
+    float uc;                // cosine of the critical angle (0 without TIR)
+    float kmax;              // k of the last row (0 when nk == 1)
+    float inv_lambda;        // 1 / absorption length + 1 / Rayleigh length [1/mm]
+    float tan_c;             // tan of the critical angle (+Inf without TIR)
+    float pde;
+    float Lx, Ly, pitch;
+    int nx, ny;
+    int images;              // add the side walls' mirror images (air-gap wrap: TIR)
+};
+
+__device__ __forceinline__ float direct_H(const DirectTable &t, float u, float k)
+{
+    float s = u > t.uc ? sqrtf((u - t.uc) / (1.0f - t.uc)) : 0.0f;
+    s = fminf(s, 1.0f);
+    const float fs = s * (float)(t.nu - 1);
+    const int j = min((int)fs, t.nu - 2);
+    const float w = fs - (float)j;
+    if (t.nk == 1) return t.H[j] + w * (t.H[j + 1] - t.H[j]);
+    const float fk = fminf(k / t.kmax, 1.0f) * (float)(t.nk - 1);
+    const int m = min((int)fk, t.nk - 2);
+    const float wk = fk - (float)m;
+    const float *a = t.H + (size_t)m * t.nu, *b = a + t.nu;
+    const float ha = a[j] + w * (a[j + 1] - a[j]);
+    const float hb = b[j] + w * (b[j + 1] - b[j]);
+    return ha + wk * (hb - ha);
+}
+
+// int over phi in [a0, a1] of [H(u_in) - H(u_out)] for the ray from (x0, y0) through the
+// rectangle [xa, xb] x [ya, yb], 8-point Gauss-Legendre
+__device__ float direct_piece(const DirectTable &t, float x0, float y0, float d, float k,
+                              float xa, float xb, float ya, float yb, float a0, float a1)
+{
+    constexpr float X[4] = {0.18343464249564980f, 0.52553240991632899f, 0.79666647741362674f,
+                            0.96028985649753623f};
+    constexpr float W[4] = {0.36268378337836198f, 0.31370664587788729f, 0.22238103445337447f,
+                            0.10122853629037626f};
+    const float mid = 0.5f * (a0 + a1), half = 0.5f * (a1 - a0);
+    float sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float phi = mid + (q < 4 ? -X[q] : X[q - 4]) * half;
+        const float w = W[q < 4 ? q : q - 4];
+        float sp, cp;
+        __sincosf(phi, &sp, &cp);
+        float tlo = 0.0f, thi = __builtin_inff();
+        if (fabsf(cp) > 1e-30f) {
+            const float t1 = (xa - x0) / cp, t2 = (xb - x0) / cp;
+            tlo = fmaxf(tlo, fminf(t1, t2));
+            thi = fminf(thi, fmaxf(t1, t2));
+        } else if (x0 < xa || x0 > xb) {
+            continue;
+        }
+        if (fabsf(sp) > 1e-30f) {
+            const float t1 = (ya - y0) / sp, t2 = (yb - y0) / sp;
+            tlo = fmaxf(tlo, fminf(t1, t2));
+            thi = fminf(thi, fmaxf(t1, t2));
+        } else if (y0 < ya || y0 > yb) {
+            continue;
+        }
+        if (!(thi > tlo)) continue;
+        const float u_in = d * rsqrtf(tlo * tlo + d * d);
+        const float u_out = thi < 1e30f ? d * rsqrtf(thi * thi + d * d) : 0.0f;
+        sum += w * (direct_H(t, u_out, k) - direct_H(t, u_in, k));
+    }
+    return sum * half;
+}
+
+// Angles (relative to phic, in (-pi, pi]) where the lines x = xa, xb and y = ya, yb cross
+// the circle of radius R about (x0, y0): there the ray's entry or exit distance passes the
+// critical-angle radius and the integrand loses smoothness.  Appends to ang[], returns the
+// new count.
+__device__ __forceinline__ int circle_crossings(float x0, float y0, float R, float xa, float xb,
+                                                float ya, float yb, float phic, float *ang, int na)
+{
+    if (!(R < 1e30f)) return na;
+    const float lines[4] = {xa - x0, xb - x0, ya - y0, yb - y0};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const float c = lines[m];
+        const float h2 = R * R - c * c;
+        if (!(h2 > 0.0f)) continue;
+        const float h = sqrtf(h2);
+#pragma unroll
+        for (int sg = -1; sg <= 1; sg += 2) {
+            float v = m < 2 ? atan2f(sg * h, c) : atan2f(c, sg * h);
+            v -= phic;
+            v = v > 3.14159265f ? v - 6.2831853f : (v < -3.14159265f ? v + 6.2831853f : v);
+            ang[na++] = v;
+        }
+    }
+    return na;
+}
+
+__device__ __forceinline__ void sort_small(float *a, int n)
+{
+    for (int i = 1; i < n; ++i)
+        for (int b = i; b > 0 && a[b - 1] > a[b]; --b) { const float v = a[b]; a[b] = a[b - 1]; a[b - 1] = v; }
+}
+
+// D_c of one cell (column-major cell index c = i + nx j) for a source at (x0, y0, z).  The
+// azimuth range splits at the cell's corner directions (the ray's entry and exit edges
+// change there) and where the cell's edges cross the critical-angle circle.
+__device__ float direct_cell(const DirectTable &t, float x0, float y0, float z, float Lz, int c)
+{
+    const float d = fmaxf(Lz - z, 1e-5f);
+    const int i = c % t.nx, j = c / t.nx;
+    const float xa = fminf(i * t.pitch, t.Lx), xb = i == t.nx - 1 ? t.Lx : fminf((i + 1) * t.pitch, t.Lx);
+    const float ya = fminf(j * t.pitch, t.Ly), yb = j == t.ny - 1 ? t.Ly : fminf((j + 1) * t.pitch, t.Ly);
+    if (!(xb > xa) || !(yb > ya)) return 0.0f;
+    // nothing beyond the critical-angle radius d tan(theta_c) lands
+    const float ex = fmaxf(fmaxf(xa - x0, x0 - xb), 0.0f), ey = fmaxf(fmaxf(ya - y0, y0 - yb), 0.0f);
+    const float R = d * t.tan_c;
+    if (ex * ex + ey * ey >= R * R) return 0.0f;
+    const float k = d * t.inv_lambda;
+    const float cx[4] = {xa, xb, xb, xa}, cy[4] = {ya, ya, yb, yb};
+    float ang[12];
+    float sum = 0.0f;
+    const bool inside = x0 >= xa && x0 <= xb && y0 >= ya && y0 <= yb;
+    // directions relative to the cell centre's (the whole circle from inside)
+    const float phic = inside ? 0.0f : atan2f(0.5f * (ya + yb) - y0, 0.5f * (xa + xb) - x0);
+    float lo = -3.14159265f, hi = 3.14159265f;
+    int na = 0;
+    for (int m = 0; m < 4; ++m) {
+        float v = atan2f(cy[m] - y0, cx[m] - x0) - phic;
+        v = v > 3.14159265f ? v - 6.2831853f : (v < -3.14159265f ? v + 6.2831853f : v);
+        ang[na++] = v;
+    }
+    if (!inside) {
+        // outside: the cell spans less than pi seen from the foot point
+        lo = fminf(fminf(ang[0], ang[1]), fminf(ang[2], ang[3]));
+        hi = fmaxf(fmaxf(ang[0], ang[1]), fmaxf(ang[2], ang[3]));
+    }
+    na = circle_crossings(x0, y0, R, xa, xb, ya, yb, phic, ang, na);
+    sort_small(ang, na);
+    if (inside) {
+        // cyclic: consecutive break angles, the last one round to the first
+        for (int m = 0; m < na; ++m) {
+            const float a0 = ang[m], a1 = m + 1 < na ? ang[m + 1] : ang[0] + 6.2831853f;
+            if (a1 > a0) sum += direct_piece(t, x0, y0, d, k, xa, xb, ya, yb, a0, a1);
+        }
+    } else {
+        for (int m = 0; m + 1 < na; ++m) {
+            const float a0 = fmaxf(ang[m], lo), a1 = fminf(ang[m + 1], hi);
+            if (a1 > a0) sum += direct_piece(t, x0, y0, d, k, xa, xb, ya, yb, phic + a0, phic + a1);
+        }
+    }
+    return t.pde * 0.0795774715f * sum;             // PDE / (4 pi)
+}
+
+// Weight of the computed terms: 1 within IMG_FULL_MM of the readout face (and, for a mirror
+// image, of its walls), 0 beyond IMG_ZERO_MM, a cubic smoothstep between.  The terms are
+// sharp only there; farther out they are smooth and left to the network, and the taper keeps
+// what the network learns smooth across the change.
+constexpr float IMG_FULL_MM = 2.0f, IMG_ZERO_MM = 3.0f;
+__device__ __forceinline__ float img_taper(float a)
+{
+    const float u = fminf(fmaxf((IMG_ZERO_MM - a) / (IMG_ZERO_MM - IMG_FULL_MM), 0.0f), 1.0f);
+    return u * u * (3.0f - 2.0f * u);
+}
+
+// The computed part of cell c's rate: the direct light plus, with t.images, the light that
+// reaches the readout face after a specular reflection off one side wall or two (total
+// internal reflection at the air gap: every ray in the readout cone meets a side wall beyond
+// the crystal-air critical angle, up to a sliver near it for n below about 1.8), i.e. the
+// direct light of the mirror sources (-x0, y0), (2Lx - x0, y0), (x0, -y0), (x0, 2Ly - y0) and
+// of the four double mirrors, each image weighted by the taper of its walls' distances.
+// Everything is weighted by the taper of the depth: the terms are sharp only near the
+// readout face (the direct light lands within d tan(theta_c) of the foot point); deeper they
+// are smooth and the network learns them with the rest.
+__device__ float direct_total(const DirectTable &t, float x0, float y0, float z, float Lz, int c)
+{
+    const float wd = img_taper(Lz - z);
+    if (wd <= 0.0f) return 0.0f;
+    float D = direct_cell(t, x0, y0, z, Lz, c);
+    if (t.images) {
+        const float wx0 = img_taper(x0), wx1 = img_taper(t.Lx - x0);
+        const float wy0 = img_taper(y0), wy1 = img_taper(t.Ly - y0);
+        const float xs[3] = {x0, -x0, 2.0f * t.Lx - x0}, wxs[3] = {1.0f, wx0, wx1};
+        const float ys[3] = {y0, -y0, 2.0f * t.Ly - y0}, wys[3] = {1.0f, wy0, wy1};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                if (a == 0 && b == 0) continue;
+                const float w = wxs[a] * wys[b];
+                if (w > 0.0f) D += w * direct_cell(t, xs[a], ys[b], z, Lz, c);
+            }
+    }
+    return wd * D;
+}
+
+// D[i][c] for n points (x, y, z) = xyz[3i..3i+2]; one thread per (point, cell)
+__global__ void direct_light_kernel(const DirectTable t, const float *xyz, int64_t n, float Lz, float *D)
+{
+    const int ncell = t.nx * t.ny;
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= n * ncell) return;
+    const int64_t i = g / ncell;
+    const int c = (int)(g % ncell);
+    D[g] = direct_total(t, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], Lz, c);
+}
+
+// ------------------------------------------------------ events from the network ----
+// Network inputs of deposits (x, y, depth feature, E): positions folded into the
+// fundamental triangle 0 <= x <= y <= L/2 of a square face when `fold` (octant code: bit 0
+// x mirrored, bit 1 y mirrored, bit 2 swapped), depth -log(Lz - z + log_depth_mm) when
+// log_depth_mm > 0; also the folded positions (for the direct light).
+__global__ void surrogate_features(const float4 *dep, const float *e_kev, int64_t n, float Lx,
+                                   float Ly, float Lz, int fold, float log_depth_mm, float *X,
+                                   float *xyz_f, uint8_t *code)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = dep[i];
+    float x = q.x, y = q.y;
+    uint8_t o = 0;
+    if (fold) {
+        const float h = 0.5f * Lx;
+        const bool mx = x > h, my = y > h;
+        if (mx) x = Lx - x;
+        if (my) y = Ly - y;
+        const bool sw = x > y;
+        if (sw) { const float v = x; x = y; y = v; }
+        o = (uint8_t)((mx ? 1 : 0) | (my ? 2 : 0) | (sw ? 4 : 0));
+    }
+    X[4 * i] = x;
+    X[4 * i + 1] = y;
+    X[4 * i + 2] = log_depth_mm > 0.0f ? -logf(Lz - q.z + log_depth_mm) : q.z;
+    X[4 * i + 3] = e_kev[i];
+    xyz_f[3 * i] = x;
+    xyz_f[3 * i + 1] = y;
+    xyz_f[3 * i + 2] = q.z;
+    code[i] = o;
+}
+
+// Event maps from per-deposit network outputs: deposit d's per-photon rate of actual cell c
+// is p = e^{Y[d][c']} + D[d][c'] (c' = inv[code_d][c], the folded index of c), its mean
+// arrival time Y[d][ncell + c'] * t_scale + t_shift after the deposit time;
+// lambda_c = sum_d yield * E_d / 1000 * p over the event's deposits with use[d] != 0, and
+// maps[e][c] ~ Poisson(lambda_c) (src/events.jl:11-22 algorithm).  One thread per
+// (event, cell).
+__global__ void sample_events_net(const float *Y, int ldy, const float *D, const uint8_t *code,
+                                  const int32_t *inv, int ncell, const float4 *dep,
+                                  const float *dep_e_kev, const uint8_t *use,
+                                  const int64_t *ev_first, int64_t nev, double yield_per_mev,
+                                  float t_shift, float t_scale, uint64_t seed, int32_t *maps,
+                                  float *tmean, float *lam_out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nev * ncell) return;
+    const int64_t e = i / ncell;
+    const int c = (int)(i % ncell);
+    double lam = 0.0, tw = 0.0;
+    for (int64_t d = ev_first[e]; d < ev_first[e + 1]; ++d) {
+        if (!use[d]) continue;
+        const int cf = code ? inv[code[d] * ncell + c] : c;
+        const float *y = Y + (size_t)d * ldy;
+        const float p = __expf(y[cf]) + (D ? D[(size_t)d * ncell + cf] : 0.0f);
+        const double mu = yield_per_mev * (double)dep_e_kev[d] / 1000.0 * (double)p;
+        lam += mu;
+        tw += mu * ((double)(y[ncell + cf] * t_scale + t_shift) + (double)dep[d].w);
+    }
+    PoisRng r;
+    r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32);
+    r.a = (uint32_t)i; r.b = (uint32_t)((uint64_t)i >> 32);
+    maps[i] = poisson_draw(r, lam);
+    if (tmean) tmean[i] = lam > 0.0 ? (float)(tw / lam) : INFINITY;
+    if (lam_out) lam_out[i] = (float)lam;
+}
+
+// Per-deposit maps of the network in actual cell order: p[d][c] and t[d][c] (ns after the
+// deposit).
+__global__ void surrogate_maps(const float *Y, int ldy, const float *D, const uint8_t *code,
+                               const int32_t *inv, int ncell, int64_t n, float t_shift,
+                               float t_scale, float *p, float *t)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n * ncell) return;
+    const int64_t d = i / ncell;
+    const int c = (int)(i % ncell);
+    const int cf = code ? inv[code[d] * ncell + c] : c;
+    const float *y = Y + (size_t)d * ldy;
+    p[i] = __expf(y[cf]) + (D ? D[(size_t)d * ncell + cf] : 0.0f);
+    if (t) t[i] = y[ncell + cf] * t_scale + t_shift;
+}
+
 }  // namespace
 
 // ================================================================= host side ====
@@ -727,7 +1056,7 @@ struct crysp_mlp {
     int64_t dloss_cap = 0;
     // the captured training step, reused while its arguments stay the same
     hipGraphExec_t gexec = nullptr;
-    const void *gX = nullptr, *gT = nullptr, *gW = nullptr;
+    const void *gX = nullptr, *gT = nullptr, *gW = nullptr, *gR0 = nullptr;
     int gbatch = 0, gsteps = 0;                     // batch size, steps captured per graph
     bool glosses = false;
     float gb1 = 0, gb2 = 0, geps = 0, gwd = 0, gmrw = 0;
@@ -805,31 +1134,35 @@ static void gemm(hipStream_t s, const GemmArgs &a, int k_slices = 1)
 }
 
 
-static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad,
-                           const int64_t *bidx = nullptr)
+// Inference on M rows of device inputs into device outputs Y [M][n_out]: one launch of the
+// row-block kernel's forward pass (RB rows per workgroup; activations stay in LDS, weights
+// stream through it).
+static int mlp_infer_dev(crysp_mlp *n, const float *Xdev, int64_t M, float *Ydev)
 {
-    const int64_t tot = (int64_t)Mpad * n->kin_pad;
-    hipLaunchKernelGGL(pack_inputs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n->stream,
-                       Xdev, M, Mpad, n->n_in, n->kin_pad, n->shift, n->scale, n->X, bidx,
-                       (int64_t)M * n->n_in);
-    const __bf16 *in = n->X;
-    int ld_in = n->kin_pad;
-    for (int l = 0; l < n->n_layers; ++l) {
-        GemmArgs a{};
-        a.M = Mpad; a.N = n->dout[l]; a.K = n->dpad[l];
-        a.A = in; a.lda = ld_in; a.a_t = 0;
-        a.B = n->Wt[l]; a.ldb = n->dpad[l]; a.b_t = 1;
-        a.bias = n->p + n->boff[l];
-        if (l + 1 < n->n_layers) {
-            a.C = n->Z[l]; a.ldc = a.N; a.H = n->H[l]; a.ldh = a.N;
-            gemm<EPI_BIAS_GELU>(n->stream, a);
-            in = n->H[l];
-            ld_in = a.N;
-        } else {
-            a.C = n->Y; a.ldc = a.N;
-            gemm<EPI_BIAS_F32>(n->stream, a);
-        }
+    const int L = n->n_layers;
+    if (L > MAXL) return sfail(CRYSP_EINVAL, "too many layers");
+    if (M > ((int64_t)1 << 31) - RB) return sfail(CRYSP_ERANGE, "more than 2^31 rows");
+    RowArgs ra{};
+    ra.L = L; ra.M = (int)M; ra.Mpad = (int)(((M + RB - 1) / RB) * RB); ra.n_in = n->n_in;
+    ra.kin_pad = n->kin_pad; ra.n_out = n->n_out; ra.n_pois = n->n_pois;
+    for (int l = 0; l < L; ++l) {
+        ra.din[l] = n->din[l]; ra.dpad[l] = n->dpad[l]; ra.dout[l] = n->dout[l];
+        ra.Wt[l] = n->Wt[l]; ra.Wn[l] = n->Wn[l]; ra.bias[l] = n->p + n->boff[l];
     }
+    ra.Xraw = Xdev; ra.shift = n->shift; ra.scale = n->scale;
+    ra.Y = Ydev;
+    const dim3 grid((unsigned)(ra.Mpad / RB));
+    switch (L) {
+    case 2: hipLaunchKernelGGL((mlp_rows<2, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 3: hipLaunchKernelGGL((mlp_rows<3, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 4: hipLaunchKernelGGL((mlp_rows<4, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 5: hipLaunchKernelGGL((mlp_rows<5, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 6: hipLaunchKernelGGL((mlp_rows<6, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 7: hipLaunchKernelGGL((mlp_rows<7, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    case 8: hipLaunchKernelGGL((mlp_rows<8, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    default: hipLaunchKernelGGL((mlp_rows<9, false>), grid, dim3(RT), 0, n->stream, ra); break;
+    }
+    SCHK(hipGetLastError());
     return CRYSP_OK;
 }
 
@@ -837,8 +1170,9 @@ static int mlp_forward_dev(crysp_mlp *n, const float *Xdev, int M, int Mpad,
 // into loss_out[0], or loss_out[ctr[0]] with loss_by_batch.  With ctr (graph replays),
 // ctr[0] is the batch index whose rows are read, and ctr advances.  Three launches:
 // mlp_rows, gemm_grouped (dW of every layer), reduce_grads.
-static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const float *W, int M,
-                        int Mpad, double wsum, float *loss_out, bool loss_by_batch, int64_t *ctr)
+static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const float *W,
+                        const float *R0, int M, int Mpad, double wsum, float *loss_out,
+                        bool loss_by_batch, int64_t *ctr)
 {
     const int L = n->n_layers;
     if (L > MAXL || Mpad % RB || Mpad > n->cap) return sfail(CRYSP_EINVAL, "bad training shape");
@@ -854,7 +1188,7 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
         ra.bpart[l] = n->bpart + n->bpart_off[l];
     }
     ra.XT = n->X; ra.lpart = n->lpart;
-    ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.shift = n->shift; ra.scale = n->scale;
+    ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.R0 = R0; ra.shift = n->shift; ra.scale = n->scale;
     ra.inv_wsum = (float)(1.0 / wsum);
     ra.bidx = ctr;
     switch (L) {                                    // the layer count is a template argument, so
@@ -1101,36 +1435,42 @@ int crysp_mlp_forward(crysp_mlp *n, const float *X, int64_t M, float *Y)
     if (!n || !X || !Y || M < 0) return sfail(CRYSP_EINVAL, "bad arguments");
     if (M == 0) return CRYSP_OK;
     SCHK(hipSetDevice(n->device));
-    const int64_t chunk = 65536;
     Staged sx;
     int rc = stage(n->stream, X, (size_t)M * n->n_in * 4, sx);
     if (rc) return rc;
-    const bool ydev = is_dev(Y);
+    if (is_dev(Y)) {
+        // device outputs: the fused forward kernel writes them in place
+        if ((rc = mlp_infer_dev(n, (const float *)sx.d, M, Y))) return rc;
+        SCHK(hipStreamSynchronize(n->stream));
+        return CRYSP_OK;
+    }
+    const int64_t chunk = 65536;
     if ((rc = mlp_alloc_batch(n, (int)std::min<int64_t>(chunk, ((M + 63) / 64) * 64)))) return rc;
     for (int64_t r0 = 0; r0 < M; r0 += chunk) {
         const int m = (int)std::min<int64_t>(chunk, M - r0);
-        const int mpad = ((m + 63) / 64) * 64;
-        if ((rc = mlp_forward_dev(n, (const float *)sx.d + r0 * n->n_in, m, mpad))) return rc;
+        if ((rc = mlp_infer_dev(n, (const float *)sx.d + r0 * n->n_in, m, n->Y))) return rc;
         SCHK(hipMemcpyAsync(Y + r0 * n->n_out, n->Y, (size_t)m * n->n_out * 4,
-                            ydev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, n->stream));
+                            hipMemcpyDeviceToHost, n->stream));
     }
     SCHK(hipStreamSynchronize(n->stream));
     return CRYSP_OK;
 }
 
 // gradient of the weighted MSE on one batch (no update): grads flat fp32, loss scalar
-int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float *W, int32_t M,
-                       float *grads, float *loss)
+int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float *W,
+                       const float *R0, int32_t M, float *grads, float *loss)
 {
     if (!n || !X || !T || M < 1) return sfail(CRYSP_EINVAL, "bad arguments");
+    if (R0 && n->n_pois < 1) return sfail(CRYSP_EINVAL, "rate offsets need Poisson outputs");
     SCHK(hipSetDevice(n->device));
     const int mpad = ((M + 63) / 64) * 64;
     int rc = mlp_alloc_batch(n, mpad);
     if (rc) return rc;
-    Staged sx, st, sw;
+    Staged sx, st, sw, sr;
     if ((rc = stage(n->stream, X, (size_t)M * n->n_in * 4, sx)) ||
         (rc = stage(n->stream, T, (size_t)M * n->n_out * 4, st)) ||
-        (rc = stage(n->stream, W, (size_t)M * n->n_out * 4, sw)))
+        (rc = stage(n->stream, W, (size_t)M * n->n_out * 4, sw)) ||
+        (rc = stage(n->stream, R0, (size_t)M * n->n_pois * 4, sr)))
         return rc;
     double wsum = (double)M * n->n_out;
     if (W) {
@@ -1140,8 +1480,8 @@ int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float
         for (float x : hw) wsum += x;
         if (!(wsum > 0.0)) return sfail(CRYSP_EINVAL, "loss weights sum to zero");
     }
-    if ((rc = mlp_grad_dev(n, (const float *)sx.d, (const float *)st.d, (const float *)sw.d, M, mpad,
-                           wsum, n->loss, false, nullptr)))
+    if ((rc = mlp_grad_dev(n, (const float *)sx.d, (const float *)st.d, (const float *)sw.d,
+                           (const float *)sr.d, M, mpad, wsum, n->loss, false, nullptr)))
         return rc;
     if (grads) SCHK(hipMemcpyAsync(grads, n->g, n->n_params * 4, hipMemcpyDeviceToHost, n->stream));
     if (loss) SCHK(hipMemcpyAsync(loss, n->loss, 4, hipMemcpyDeviceToHost, n->stream));
@@ -1153,13 +1493,15 @@ int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float
 // [b*batch, (b+1)*batch) of X, T, W (the caller shuffles between epochs).  The loss of
 // every step is written to losses[b] (optional).
 int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const float *W,
-                            int64_t n_batches, int32_t batch, float lr, float beta1, float beta2,
-                            float eps, float weight_decay, float mean_row_weight, float *losses)
+                            const float *R0, int64_t n_batches, int32_t batch, float lr,
+                            float beta1, float beta2, float eps, float weight_decay,
+                            float mean_row_weight, float *losses)
 {
     if (!n || !X || !T || n_batches < 0 || batch < 64 || batch % 64)
         return sfail(CRYSP_EINVAL, "bad arguments (batch must be a multiple of 64)");
-    if (!is_dev(X) || !is_dev(T) || (W && !is_dev(W)))
+    if (!is_dev(X) || !is_dev(T) || (W && !is_dev(W)) || (R0 && !is_dev(R0)))
         return sfail(CRYSP_EINVAL, "training arrays must be device-resident");
+    if (R0 && n->n_pois < 1) return sfail(CRYSP_EINVAL, "rate offsets need Poisson outputs");
     SCHK(hipSetDevice(n->device));
     int rc = mlp_alloc_batch(n, batch);
     if (rc) return rc;
@@ -1180,10 +1522,11 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
     // launch per S steps.  The graph is kept for later calls with the same arrays, batch,
     // step count per graph and optimizer settings (an epoch loop over reshuffled copies in
     // the same buffers).
-    auto enqueue_step = [&](const float *x, const float *t, const float *w, float *lout,
-                            int64_t *ctr, const float *dlr, float c1, float c2) -> int {
+    auto enqueue_step = [&](const float *x, const float *t, const float *w, const float *r0,
+                            float *lout, int64_t *ctr, const float *dlr, float c1, float c2) -> int {
         int r;
-        if ((r = mlp_grad_dev(n, x, t, w, batch, batch, wsum, lout ? lout : n->loss, lout != nullptr, ctr)))
+        if ((r = mlp_grad_dev(n, x, t, w, r0, batch, batch, wsum, lout ? lout : n->loss,
+                              lout != nullptr, ctr)))
             return r;
         mlp_adam(n, lr, beta1, beta2, eps, c1, c2, weight_decay, ctr ? ctr + 1 : nullptr, dlr);
         return CRYSP_OK;
@@ -1197,7 +1540,8 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
         n->hctr[1] = n->step;
         SCHK(hipMemcpyAsync(n->ctr, n->hctr, sizeof n->hctr, hipMemcpyHostToDevice, n->stream));
         SCHK(hipMemcpyAsync(n->dlr, &lr, sizeof lr, hipMemcpyHostToDevice, n->stream));
-        const bool same = n->gexec && n->gX == X && n->gT == T && n->gW == W && n->gbatch == batch &&
+        const bool same = n->gexec && n->gX == X && n->gT == T && n->gW == W && n->gR0 == R0 &&
+                          n->gbatch == batch &&
                           n->gsteps == steps &&
                           n->glosses == (dl != nullptr) && n->gb1 == beta1 && n->gb2 == beta2 &&
                           n->geps == eps && n->gwd == weight_decay && n->gmrw == mean_row_weight;
@@ -1207,7 +1551,7 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(n->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
                 for (int k = 0; k < steps && rc == CRYSP_OK; ++k)
-                    rc = enqueue_step(X, T, W, dl, n->ctr, n->dlr, 0.0f, 0.0f);
+                    rc = enqueue_step(X, T, W, R0, dl, n->ctr, n->dlr, 0.0f, 0.0f);
                 const hipError_t e = hipStreamEndCapture(n->stream, &g);
                 if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
                 if (e != hipSuccess || !g || hipGraphInstantiate(&n->gexec, g, nullptr, nullptr, 0) != hipSuccess)
@@ -1215,7 +1559,7 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
                 if (g) (void)hipGraphDestroy(g);
             }
             if (!n->gexec) (void)hipGetLastError();     // capture unavailable: launch eagerly
-            n->gX = X; n->gT = T; n->gW = W; n->gbatch = batch; n->gsteps = steps; n->glosses = dl != nullptr;
+            n->gX = X; n->gT = T; n->gW = W; n->gR0 = R0; n->gbatch = batch; n->gsteps = steps; n->glosses = dl != nullptr;
             n->gb1 = beta1; n->gb2 = beta2; n->geps = eps; n->gwd = weight_decay; n->gmrw = mean_row_weight;
         }
         graph = n->gexec != nullptr;
@@ -1228,10 +1572,11 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
             const float *x = X + (size_t)b * batch * n->n_in;
             const float *t = T + (size_t)b * batch * n->n_out;
             const float *w = W ? W + (size_t)b * batch * n->n_out : nullptr;
+            const float *r0 = R0 ? R0 + (size_t)b * batch * n->n_pois : nullptr;
             n->step += 1;
             const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)n->step));
             const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)n->step));
-            if ((rc = enqueue_step(x, t, w, dl ? dl + b : nullptr, nullptr, nullptr, c1, c2))) return rc;
+            if ((rc = enqueue_step(x, t, w, r0, dl ? dl + b : nullptr, nullptr, nullptr, c1, c2))) return rc;
         }
     }
     if (dl) SCHK(hipMemcpyAsync(losses, dl, (size_t)n_batches * 4, hipMemcpyDeviceToHost, n->stream));
@@ -1309,6 +1654,287 @@ int crysp_debug_gemm(int device, int32_t M, int32_t N, int32_t K, const uint16_t
     SCHK(hipGetLastError());
     SCHK(hipMemcpy(C, c, (size_t)M * N * 4, hipMemcpyDeviceToHost));
     (void)hipFree(a); (void)hipFree(b); (void)hipFree(c);
+    return CRYSP_OK;
+}
+
+}  // extern "C"
+
+// =========================================================== surrogate object ====
+
+// Fresnel transmission crystal -> coupling at incidence cosine v (src/optics.jl:10-19 in
+// binary64): zero at and beyond total internal reflection
+static double fresnel_T64(double n1, double n2, double v)
+{
+    const double q = n1 / n2;
+    const double s2 = q * q * (1.0 - v * v);
+    if (s2 >= 1.0) return 0.0;
+    const double ct = std::sqrt(1.0 - s2);
+    const double rs = (n1 * v - n2 * ct) / (n1 * v + n2 * ct);
+    const double rp = (n1 * ct - n2 * v) / (n1 * ct + n2 * v);
+    return 1.0 - 0.5 * (rs * rs + rp * rp);
+}
+
+struct crysp_surrogate {
+    crysp_mlp *net = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    crysp_box box{};
+    crysp_surrogate_config cfg{};
+    int ncell = 0;
+    DirectTable tab{};
+    float *dH = nullptr;                            // the H table on the device
+    int32_t *dinv = nullptr;                        // [8][ncell] folded index of actual cell c
+    // per-deposit buffers, grown on demand
+    int64_t cap = 0;
+    float *X = nullptr, *xyzf = nullptr, *D = nullptr, *Y = nullptr;
+    uint8_t *code = nullptr;
+};
+
+static void sur_free_bufs(crysp_surrogate *s)
+{
+    auto fre = [](void *q) { if (q) (void)hipFree(q); };
+    fre(s->X); fre(s->xyzf); fre(s->D); fre(s->Y); fre(s->code);
+    s->X = s->xyzf = s->D = s->Y = nullptr;
+    s->code = nullptr;
+    s->cap = 0;
+}
+
+static int sur_reserve(crysp_surrogate *s, int64_t n)
+{
+    if (n <= s->cap) return CRYSP_OK;
+    sur_free_bufs(s);
+    const int64_t want = std::max<int64_t>(n, 1024);
+    if (hipMalloc(&s->X, (size_t)want * 16) != hipSuccess ||
+        hipMalloc(&s->xyzf, (size_t)want * 12) != hipSuccess ||
+        hipMalloc(&s->D, (size_t)want * s->ncell * 4) != hipSuccess ||
+        hipMalloc(&s->Y, (size_t)want * s->net->n_out * 4) != hipSuccess ||
+        hipMalloc(&s->code, (size_t)want) != hipSuccess) {
+        (void)hipGetLastError();
+        sur_free_bufs(s);
+        return sfail(CRYSP_ENOMEM, "surrogate buffers: allocation failed");
+    }
+    s->cap = want;
+    return CRYSP_OK;
+}
+
+// features, direct light and the network for n deposits (device), into s->X, s->D, s->Y
+static int sur_evaluate(crysp_surrogate *s, const float4 *dep, const float *e_kev, int64_t n)
+{
+    int rc = sur_reserve(s, n);
+    if (rc) return rc;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(surrogate_features, dim3(nb), dim3(256), 0, s->stream, dep, e_kev, n,
+                       s->box.L[0], s->box.L[1], s->box.L[2], (int)s->cfg.fold, s->cfg.log_depth_mm,
+                       s->X, s->xyzf, s->code);
+    if (s->cfg.direct) {
+        const int64_t tot = n * s->ncell;
+        hipLaunchKernelGGL(direct_light_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                           s->stream, s->tab, (const float *)s->xyzf, n, s->box.L[2], s->D);
+    }
+    SCHK(hipGetLastError());
+    // the network on the library's own MLP stream: order the two streams through a sync
+    SCHK(hipStreamSynchronize(s->stream));
+    return crysp_mlp_forward(s->net, s->X, n, s->Y);
+}
+
+extern "C" {
+
+crysp_surrogate *crysp_surrogate_create(crysp_mlp *net, const crysp_box *box,
+                                        const crysp_optical_params *op, const crysp_readout *ro,
+                                        const crysp_surrogate_config *cfg)
+{
+    if (!net || !box || !op || !ro || !cfg) { sfail(CRYSP_EINVAL, "null argument"); return nullptr; }
+    const int ncell = ro->grid.nx * ro->grid.ny;
+    if (net->n_in != 4 || net->n_out < 2 * ncell) {
+        sfail(CRYSP_EINVAL, "the network needs 4 inputs and 2 * nx * ny outputs");
+        return nullptr;
+    }
+    if (cfg->fold && (ro->grid.nx != ro->grid.ny || box->L[0] != box->L[1])) {
+        sfail(CRYSP_EINVAL, "folding needs a square face and a square grid");
+        return nullptr;
+    }
+    if (cfg->direct && (ro->disc || ro->two_sided)) {
+        sfail(CRYSP_EINVAL, "the direct-light term covers a full single readout face");
+        return nullptr;
+    }
+    if (hipSetDevice(net->device) != hipSuccess) { sfail(CRYSP_EHIP, "hipSetDevice"); return nullptr; }
+    auto *s = new crysp_surrogate();
+    s->net = net;
+    s->device = net->device;
+    s->box = *box;
+    s->cfg = *cfg;
+    s->ncell = ncell;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        sfail(CRYSP_EHIP, "stream creation failed");
+        return nullptr;
+    }
+    // folded index of every actual cell for the 8 octant codes (column-major cells
+    // i + n j; bit 0 mirrors i, bit 1 mirrors j, bit 2 swaps them)
+    std::vector<int32_t> inv((size_t)8 * ncell, 0);
+    const int nn = ro->grid.nx;
+    for (int o = 0; o < 8; ++o)
+        for (int c = 0; c < ncell; ++c) {
+            if (!cfg->fold) { inv[o * ncell + c] = c; continue; }
+            const int a1 = c % nn, b1 = c / nn;     // folded cell (a1, b1) -> actual (i, j)
+            int a = a1, b = b1;
+            if (o & 4) { a = b1; b = a1; }
+            const int i = (o & 1) ? nn - 1 - a : a, j = (o & 2) ? nn - 1 - b : b;
+            inv[o * ncell + (i + nn * j)] = c;
+        }
+    // H(u, k) = int_u^1 T(v) e^{-k/v} dv on u = uc + (1 - uc) s^2 (which takes the square-root
+    // onset of T at the critical angle), cumulative Simpson from s = 1 down
+    const double n1 = op->n_crystal, n2 = op->n_coupling;
+    const double uc = n1 > n2 ? std::sqrt(1.0 - (n2 / n1) * (n2 / n1)) : 0.0;
+    double inv_lambda = 0.0;
+    if (std::isfinite(op->abs_len_mm)) inv_lambda += 1.0 / op->abs_len_mm;
+    if (std::isfinite(op->rayleigh_mm)) inv_lambda += 1.0 / op->rayleigh_mm;
+    const double kmax = box->L[2] * inv_lambda;
+    const int nu = 1025, nk = kmax > 0.0 ? 33 : 1, sub = 16;
+    std::vector<float> H((size_t)nu * nk);
+    for (int m = 0; m < nk; ++m) {
+        const double k = nk == 1 ? 0.0 : kmax * m / (nk - 1);
+        auto f = [&](double sv) {                   // integrand in s: T(v) e^{-k/v} dv/ds
+            const double v = uc + (1.0 - uc) * sv * sv;
+            if (v <= 0.0) return 0.0;
+            return fresnel_T64(n1, n2, v) * std::exp(-k / v) * 2.0 * (1.0 - uc) * sv;
+        };
+        double acc = 0.0;
+        H[(size_t)m * nu + nu - 1] = 0.0f;
+        for (int j = nu - 1; j > 0; --j) {
+            const double s1 = (double)j / (nu - 1), s0 = (double)(j - 1) / (nu - 1);
+            const double h = (s1 - s0) / sub;
+            double part = 0.0;
+            for (int q = 0; q < sub; ++q) {
+                const double a = s0 + q * h;
+                part += h / 6.0 * (f(a) + 4.0 * f(a + 0.5 * h) + f(a + h));
+            }
+            acc += part;
+            H[(size_t)m * nu + j - 1] = (float)acc;
+        }
+    }
+    s->tab.nu = nu; s->tab.nk = nk; s->tab.uc = (float)uc; s->tab.kmax = (float)kmax;
+    s->tab.inv_lambda = (float)inv_lambda;
+    s->tab.tan_c = uc > 0.0 ? (float)(std::sqrt(1.0 - uc * uc) / uc) : INFINITY;
+    if (n1 <= n2) s->tab.tan_c = INFINITY;
+    s->tab.pde = op->pde;
+    s->tab.Lx = box->L[0]; s->tab.Ly = box->L[1]; s->tab.pitch = ro->grid.pitch_mm;
+    s->tab.nx = ro->grid.nx; s->tab.ny = ro->grid.ny;
+    s->tab.images = cfg->images && op->air_gap;
+    if (hipMalloc(&s->dH, H.size() * 4) != hipSuccess ||
+        hipMalloc(&s->dinv, inv.size() * 4) != hipSuccess ||
+        hipMemcpy(s->dH, H.data(), H.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->dinv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        crysp_surrogate_destroy(s);
+        sfail(CRYSP_ENOMEM, "surrogate tables: allocation failed");
+        return nullptr;
+    }
+    s->tab.H = s->dH;
+    return s;
+}
+
+void crysp_surrogate_destroy(crysp_surrogate *s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    sur_free_bufs(s);
+    if (s->dH) (void)hipFree(s->dH);
+    if (s->dinv) (void)hipFree(s->dinv);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int crysp_surrogate_direct_light(crysp_surrogate *s, const float *xyz, int64_t n, float *D)
+{
+    if (!s || !xyz || !D || n < 0) return sfail(CRYSP_EINVAL, "bad arguments");
+    if (n == 0) return CRYSP_OK;
+    SCHK(hipSetDevice(s->device));
+    Staged sx;
+    int rc = stage(s->stream, xyz, (size_t)n * 12, sx);
+    if (rc) return rc;
+    const bool ddev = is_dev(D);
+    float *dD = D;
+    if (!ddev) SCHK(hipMalloc(&dD, (size_t)n * s->ncell * 4));
+    const int64_t tot = n * s->ncell;
+    hipLaunchKernelGGL(direct_light_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s->stream,
+                       s->tab, (const float *)sx.d, n, s->box.L[2], dD);
+    SCHK(hipGetLastError());
+    if (!ddev) SCHK(hipMemcpyAsync(D, dD, (size_t)tot * 4, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    if (!ddev) (void)hipFree(dD);
+    return CRYSP_OK;
+}
+
+int crysp_surrogate_predict(crysp_surrogate *s, const float *dep_xyzt, const float *e_kev, int64_t n,
+                            float *p, float *t)
+{
+    if (!s || !dep_xyzt || !e_kev || !p || n < 0) return sfail(CRYSP_EINVAL, "bad arguments");
+    if (n == 0) return CRYSP_OK;
+    SCHK(hipSetDevice(s->device));
+    Staged sd, se;
+    int rc;
+    if ((rc = stage(s->stream, dep_xyzt, (size_t)n * 16, sd)) ||
+        (rc = stage(s->stream, e_kev, (size_t)n * 4, se)))
+        return rc;
+    if ((rc = sur_evaluate(s, (const float4 *)sd.d, (const float *)se.d, n))) return rc;
+    const size_t cells = (size_t)n * s->ncell;
+    const bool pdev = is_dev(p), tdev = is_dev(t);
+    float *dp = p, *dt = t;
+    if (!pdev) SCHK(hipMalloc(&dp, cells * 4));
+    if (t && !tdev) SCHK(hipMalloc(&dt, cells * 4));
+    hipLaunchKernelGGL(surrogate_maps, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->stream,
+                       (const float *)s->Y, s->net->n_out, s->cfg.direct ? (const float *)s->D : nullptr,
+                       s->cfg.fold ? (const uint8_t *)s->code : nullptr, (const int32_t *)s->dinv,
+                       s->ncell, n, s->cfg.t_shift, s->cfg.t_scale, dp, dt);
+    SCHK(hipGetLastError());
+    if (!pdev) SCHK(hipMemcpyAsync(p, dp, cells * 4, hipMemcpyDeviceToHost, s->stream));
+    if (t && !tdev) SCHK(hipMemcpyAsync(t, dt, cells * 4, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    if (!pdev) (void)hipFree(dp);
+    if (t && !tdev) (void)hipFree(dt);
+    return CRYSP_OK;
+}
+
+int crysp_surrogate_sample(crysp_surrogate *s, const float *dep_xyzt, const float *dep_e_kev,
+                           const uint8_t *use, const int64_t *ev_first, int64_t ndep, int64_t nev,
+                           double yield_per_mev, uint64_t seed, int32_t *maps, float *tmean,
+                           float *lam)
+{
+    if (!s || !ev_first || !maps || nev < 0 || ndep < 0 || (ndep > 0 && (!dep_xyzt || !dep_e_kev || !use)))
+        return sfail(CRYSP_EINVAL, "bad arguments");
+    if (nev == 0) return CRYSP_OK;
+    SCHK(hipSetDevice(s->device));
+    Staged sd, se, su, sf;
+    int rc;
+    if ((rc = stage(s->stream, dep_xyzt, (size_t)ndep * 16, sd)) ||
+        (rc = stage(s->stream, dep_e_kev, (size_t)ndep * 4, se)) ||
+        (rc = stage(s->stream, use, (size_t)ndep, su)) ||
+        (rc = stage(s->stream, ev_first, (size_t)(nev + 1) * 8, sf)))
+        return rc;
+    if (ndep > 0 && (rc = sur_evaluate(s, (const float4 *)sd.d, (const float *)se.d, ndep))) return rc;
+    const size_t cells = (size_t)nev * s->ncell;
+    const bool mdev = is_dev(maps), tdev = is_dev(tmean), ldev = is_dev(lam);
+    int32_t *dm = maps;
+    float *dt = tmean, *dl = lam;
+    if (!mdev) SCHK(hipMalloc(&dm, cells * 4));
+    if (tmean && !tdev) SCHK(hipMalloc(&dt, cells * 4));
+    if (lam && !ldev) SCHK(hipMalloc(&dl, cells * 4));
+    hipLaunchKernelGGL(sample_events_net, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->stream,
+                       (const float *)s->Y, s->net->n_out, s->cfg.direct ? (const float *)s->D : nullptr,
+                       s->cfg.fold ? (const uint8_t *)s->code : nullptr, (const int32_t *)s->dinv,
+                       s->ncell, (const float4 *)sd.d, (const float *)se.d, (const uint8_t *)su.d,
+                       (const int64_t *)sf.d, nev, yield_per_mev, s->cfg.t_shift, s->cfg.t_scale, seed,
+                       dm, dt, dl);
+    SCHK(hipGetLastError());
+    if (!mdev) SCHK(hipMemcpyAsync(maps, dm, cells * 4, hipMemcpyDeviceToHost, s->stream));
+    if (tmean && !tdev) SCHK(hipMemcpyAsync(tmean, dt, cells * 4, hipMemcpyDeviceToHost, s->stream));
+    if (lam && !ldev) SCHK(hipMemcpyAsync(lam, dl, cells * 4, hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    if (!mdev) (void)hipFree(dm);
+    if (tmean && !tdev) (void)hipFree(dt);
+    if (lam && !ldev) (void)hipFree(dl);
     return CRYSP_OK;
 }
 

@@ -1237,6 +1237,113 @@ done:
     return CRYSP_OK;
 }
 
+}  // extern "C"
+
+// The launch-knob sweep shared by the tuners: every (wg, ppt, chunk, refill_min) of the
+// lists, `reps` timed calls each (best of reps kept), the fastest combination left in the
+// context.  run() makes one call on the context's own device buffers.
+template <class Run>
+static int autotune_sweep(crysp_ctx *c, const int32_t *wgs, int32_t n_wg, const int32_t *ppts,
+                          int32_t n_ppt, const int32_t *chunks, int32_t n_chunk,
+                          const int32_t *refills, int32_t n_refill, int32_t reps, float *times_ms,
+                          int32_t *best4, Run run)
+{
+    if (!wgs || !ppts || !chunks || !refills || n_wg < 1 || n_ppt < 1 || n_chunk < 1 ||
+        n_refill < 1 || reps < 1)
+        return fail(CRYSP_EINVAL, "bad autotune arguments");
+    const int old_wg = c->wg, old_ppt = c->ppt, old_chunk = c->chunk, old_refill = c->refill_min;
+    const bool old_timing = c->timing;
+    const double old_ms = c->kernel_ms;
+    const int64_t old_n = c->n_launches;
+    float best = 1e30f;
+    int bw = old_wg, bp = old_ppt, bc = old_chunk, br = old_refill;
+    int rc = CRYSP_OK;
+    c->timing = true;
+    int64_t idx = 0;
+    for (int a = 0; a < n_wg && rc == CRYSP_OK; ++a)
+        for (int b = 0; b < n_ppt && rc == CRYSP_OK; ++b)
+            for (int q = 0; q < n_chunk && rc == CRYSP_OK; ++q)
+                for (int r = 0; r < n_refill && rc == CRYSP_OK; ++r, ++idx) {
+                    if (wgs[a] >= 64 && wgs[a] <= 1024 &&
+                        ring_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
+                        if (times_ms) times_ms[idx] = INFINITY;     // does not fit: skipped
+                        continue;
+                    }
+                    if ((rc = crysp_set_schedule(c, wgs[a], ppts[b])) ||
+                        (rc = crysp_set_queue(c, chunks[q], refills[r])))
+                        break;
+                    float tbest = 1e30f;
+                    for (int k = 0; k < reps && rc == CRYSP_OK; ++k) {
+                        c->kernel_ms = 0.0;
+                        c->n_launches = 0;
+                        if ((rc = run())) break;
+                        tbest = std::min(tbest, (float)c->kernel_ms);
+                    }
+                    if (rc) break;
+                    if (times_ms) times_ms[idx] = tbest;
+                    if (tbest < best) { best = tbest; bw = wgs[a]; bp = ppts[b]; bc = chunks[q]; br = refills[r]; }
+                }
+    c->timing = old_timing;
+    c->kernel_ms = old_ms;
+    c->n_launches = old_n;
+    if (rc) {
+        c->wg = old_wg; c->ppt = old_ppt; c->chunk = old_chunk; c->refill_min = old_refill;
+        return rc;
+    }
+    c->wg = bw; c->ppt = bp; c->chunk = bc; c->refill_min = br;
+    if (best4) { best4[0] = bw; best4[1] = bp; best4[2] = bc; best4[3] = br; }
+    return CRYSP_OK;
+}
+
+extern "C" {
+
+int crysp_autotune_events_queue(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                                const crysp_readout *ro, const float *dep_xyzt,
+                                const int32_t *dep_nph, const int32_t *dep_event, int64_t ndep,
+                                int64_t nev, uint64_t seed, int32_t maxb, const crysp_emission *em,
+                                const int32_t *wgs, int32_t n_wg, const int32_t *ppts, int32_t n_ppt,
+                                const int32_t *chunks, int32_t n_chunk, const int32_t *refills,
+                                int32_t n_refill, int32_t reps, float *times_ms, int32_t *best4)
+{
+    if (!c || !ro) return fail(CRYSP_EINVAL, "bad autotune arguments");
+    const int nxy = ro->grid.nx * ro->grid.ny;
+    const size_t cells = (size_t)std::max<int64_t>(1, nev) * std::max(1, nxy);
+    void *dm, *dt, *dn;
+    int rc;
+    // outputs go to device scratch so the timing is the kernel's own
+    if ((rc = ensure(c, B_MISC, cells * 4, &dm)) || (rc = ensure(c, B_MISC2, cells * 4, &dt)) ||
+        (rc = ensure(c, B_NPE, (size_t)std::max<int64_t>(1, nev) * 4, &dn)))
+        return rc;
+    return autotune_sweep(c, wgs, n_wg, ppts, n_ppt, chunks, n_chunk, refills, n_refill, reps,
+                          times_ms, best4, [&]() {
+        return events_impl(c, box, op, ro, dep_xyzt, dep_nph, dep_event, ndep, nev, seed, 0, maxb,
+                           em, (int32_t *)dm, (float *)dt, (int32_t *)dn, nullptr, false);
+    });
+}
+
+int crysp_autotune_accumulate(crysp_ctx *c, const crysp_box *box, const crysp_optical_params *op,
+                              const crysp_readout *ro, const crysp_time_binning *tb, const float *px,
+                              const float *py, const float *pz, const float *pt, int32_t npos,
+                              int64_t n, uint64_t seed, int32_t maxb, const crysp_emission *em,
+                              const int32_t *wgs, int32_t n_wg, const int32_t *ppts, int32_t n_ppt,
+                              const int32_t *chunks, int32_t n_chunk, const int32_t *refills,
+                              int32_t n_refill, int32_t reps, float *times_ms, int32_t *best4)
+{
+    if (!c || !ro || !tb || tb->nbins < 0) return fail(CRYSP_EINVAL, "bad autotune arguments");
+    const int64_t nxy = (int64_t)ro->grid.nx * ro->grid.ny;
+    void *dc, *df;
+    int rc;
+    if ((rc = ensure(c, B_MISC, (size_t)nxy * (tb->nbins + 1) * 4, &dc)) ||
+        (rc = ensure(c, B_MISC2, (size_t)nxy * 4, &df)))
+        return rc;
+    int64_t ctr[CRYSP_NCOUNTERS];
+    return autotune_sweep(c, wgs, n_wg, ppts, n_ppt, chunks, n_chunk, refills, n_refill, reps,
+                          times_ms, best4, [&]() {
+        return crysp_transport_accumulate(c, box, op, ro, tb, px, py, pz, pt, npos, n, seed, 0, maxb,
+                                          em, (uint32_t *)dc, (float *)df, ctr);
+    });
+}
+
 int crysp_debug_draws(crysp_ctx *c, uint64_t seed, uint64_t pid0, int64_t n_pids,
                       int32_t n_draws, uint32_t *out)
 {

@@ -281,6 +281,32 @@ def autotune(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_n
     return (bw.value, bp.value), times
 
 
+def autotune_queue(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_ns: float,
+                   tau_rise_ns: float = 0.0, max_bounces: int = 100_000, ctx=None,
+                   wg_sizes=(128, 256, 512), ppts=(4,), chunks=(32, 64, 128, 256),
+                   refill_mins=(6, 12, 24), reps: int = 2):
+    """Sweep every launch knob -- workgroup size, photons per thread, the work queue's grab
+    size (chunk) and refill batch (refill_min) -- on this deposit table (event mode) and keep
+    the fastest in the context.  Returns (best (wg, ppt, chunk, refill_min),
+    times_ms[n_wg, n_ppt, n_chunk, n_refill])."""
+    ctx = ctx or _lib.default_context()
+    ro = readout(ro) if isinstance(ro, SipmGrid) else ro
+    lists = [np.ascontiguousarray(v, np.int32) for v in (wg_sizes, ppts, chunks, refill_mins)]
+    times = np.zeros(tuple(v.size for v in lists), np.float32)
+    best = np.zeros(4, np.int32)
+    em = Emission(tau_ns, tau_rise_ns)
+    xyzt = np.ascontiguousarray(dep.xyzt, np.float32)
+    nph = np.ascontiguousarray(dep.nph, np.int32)
+    ev = np.ascontiguousarray(dep.event, np.int32)
+    rc = _lib.load().crysp_autotune_events_queue(
+        ctx.handle, C.byref(box), C.byref(op), C.byref(ro), xyzt.ctypes.data, nph.ctypes.data,
+        ev.ctypes.data, xyzt.shape[0], dep.n_events, int(seed), int(max_bounces), C.byref(em),
+        *[a for v in lists for a in (v.ctypes.data, v.size)], int(reps), times.ctypes.data,
+        best.ctypes.data)
+    _lib.check(rc, "crysp_autotune_events_queue")
+    return tuple(int(v) for v in best), times
+
+
 def transport_events_multibox(boxes, dep_box, op: OpticalParams, ro, dep: Deposits, *, seed: int,
                               tau_ns: float, tau_rise_ns: float = 0.0, pid0: int = 0,
                               max_bounces: int = 100_000, ctx=None, device_arrays=None,

@@ -12,11 +12,20 @@ evenly) and the mean arrival time of a detected photon after the deposit (output
 64..127, standardized, squared error weighted by the counts); the mean count map is
 Y E / 1000 * p.  Energy enters as the fourth input.
 
-Inference.  `predict` runs the network (BF16 MFMA kernels).  For event sampling the
-network is evaluated once on a regular grid over the crystal (`bake`) and deposits
-interpolate it trilinearly on the device, so the per-event cost no longer includes an
-MLP evaluation (0.33 MFLOP per deposit).  Deposits within `fallback_mm` of a face go to
-the Monte Carlo instead (`split_deposits`).
+Computed light.  Near the readout face the maps are sharp: light that reaches the face
+straight from the deposit lands within d tan(theta_c) of the foot point, so a cell's share
+jumps across cell boundaries on the scale of the depth d.  That part is computed, not
+learned: the direct light (Fresnel transmission into the coupling, bulk attenuation,
+PDE, integrated over each cell) and, behind an air gap, its mirror images in the side
+walls (total internal reflection), weighted by a smoothstep that is 1 within 2 mm of the
+readout face and 0 beyond 3 mm.  The per-photon rate of cell c is e^{y_c} + D_c; the
+network learns e^{y_c}, which is smooth (training uses D as a rate offset in the Poisson
+loss).
+
+Inference.  `predict` and `sample` run the whole chain on the device: input features and
+symmetry folding, the computed light, the network (one fused BF16 MFMA kernel) and, for
+events, the per-event Poisson sampling.  `bake` and `sample_events` keep the older path
+that interpolates a baked grid of the network.
 """
 from __future__ import annotations
 
@@ -48,10 +57,18 @@ def _bind():
     lib.crysp_mlp_set_input_norm.argtypes = [vp, vp, vp]
     lib.crysp_mlp_set_poisson_outputs.argtypes = [vp, C.c_int32]
     lib.crysp_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
-    lib.crysp_mlp_gradient.argtypes = [vp, vp, vp, vp, C.c_int32, vp, vp]
-    lib.crysp_mlp_train_batches.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int32, C.c_float,
+    lib.crysp_mlp_gradient.argtypes = [vp, vp, vp, vp, vp, C.c_int32, vp, vp]
+    lib.crysp_mlp_train_batches.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_int32, C.c_float,
                                             C.c_float, C.c_float, C.c_float, C.c_float,
                                             C.c_float, vp]
+    lib.crysp_surrogate_create.restype = vp
+    lib.crysp_surrogate_create.argtypes = [vp, C.POINTER(Box), C.POINTER(OpticalParams), vp,
+                                           C.POINTER(SurrogateConfig)]
+    lib.crysp_surrogate_destroy.argtypes = [vp]
+    lib.crysp_surrogate_direct_light.argtypes = [vp, vp, C.c_int64, vp]
+    lib.crysp_surrogate_predict.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+    lib.crysp_surrogate_sample.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_double,
+                                           C.c_uint64, vp, vp, vp]
     lib.crysp_surrogate_sample_events.argtypes = [C.c_int, vp, C.c_int32, C.c_int32, C.c_int32,
                                                   C.c_int32, C.POINTER(Box), vp, vp, vp, vp,
                                                   C.c_int64, C.c_int64, C.c_double, C.c_uint64,
@@ -63,6 +80,12 @@ def _bind():
                                      C.POINTER(Emission), vp, vp]
     lib._mlp_bound = True
     return lib
+
+
+class SurrogateConfig(C.Structure):
+    """crysp_surrogate_config (include/crysplight.h)."""
+    _fields_ = [("fold", C.c_uint8), ("direct", C.c_uint8), ("images", C.c_uint8), ("pad_", C.c_uint8),
+                ("log_depth_mm", C.c_float), ("t_shift", C.c_float), ("t_scale", C.c_float)]
 
 
 def _mcheck(rc, what):
@@ -123,6 +146,7 @@ class MLP:
 
     def set_poisson_outputs(self, n_pois: int):
         _mcheck(_bind().crysp_mlp_set_poisson_outputs(self._h, int(n_pois)), "set_poisson_outputs")
+        self.n_pois = int(n_pois)
 
     def forward(self, X, out=None):
         n = int(X.shape[0])
@@ -137,31 +161,39 @@ class MLP:
         _mcheck(_bind().crysp_mlp_forward(self._h, _ptr(X), n, _ptr(out)), "forward")
         return out
 
-    def gradient(self, X, T, W=None):
+    def gradient(self, X, T, W=None, R0=None):
+        """Loss and gradient of one batch (host arrays); R0: optional rate offsets
+        [rows x n_pois] (the Poisson rate is e^y + R0)."""
         X = np.ascontiguousarray(X, np.float32)
         T = np.ascontiguousarray(T, np.float32)
         W = None if W is None else np.ascontiguousarray(W, np.float32)
+        R0 = None if R0 is None else np.ascontiguousarray(R0, np.float32)
         g = np.zeros(self.n_params, np.float32)
         loss = np.zeros(1, np.float32)
         _mcheck(_bind().crysp_mlp_gradient(self._h, X.ctypes.data, T.ctypes.data,
-                                           None if W is None else W.ctypes.data, X.shape[0],
+                                           None if W is None else W.ctypes.data,
+                                           None if R0 is None else R0.ctypes.data, X.shape[0],
                                            g.ctypes.data, loss.ctypes.data), "gradient")
         return float(loss[0]), g
 
     def train_batches(self, X, T, W, n_batches, batch, lr=1e-3, betas=(0.9, 0.999), eps=1e-8,
-                      weight_decay=0.0, mean_row_weight=1.0):
-        """Adam steps over consecutive rows of device tensors X, T, W."""
+                      weight_decay=0.0, mean_row_weight=1.0, R0=None):
+        """Adam steps over consecutive rows of device tensors X, T, W (and the rate offsets
+        R0, rows of n_pois, when given)."""
         losses = np.zeros(max(1, n_batches), np.float32)
         rows = int(n_batches) * int(batch)
-        for a, name, cols in ((X, "X", self.n_in), (T, "T", self.n_out), (W, "W", self.n_out)):
-            if a is None and name == "W":
+        n_pois = getattr(self, "n_pois", 0)
+        for a, name, cols in ((X, "X", self.n_in), (T, "T", self.n_out), (W, "W", self.n_out),
+                              (R0, "R0", n_pois)):
+            if a is None and name in ("W", "R0"):
                 continue
             _lib.check_array(a, name, np.float32)
             if a.ndim != 2 or a.shape[1] != cols or a.shape[0] < rows:
                 raise ValueError(f"{name}: shape {tuple(a.shape)}, expected at least "
                                  f"({rows}, {cols})")
-        _lib.ready(*(a for a in (X, T, W) if a is not None))
-        _mcheck(_bind().crysp_mlp_train_batches(self._h, _ptr(X), _ptr(T), _ptr(W), int(n_batches),
+        _lib.ready(*(a for a in (X, T, W, R0) if a is not None))
+        _mcheck(_bind().crysp_mlp_train_batches(self._h, _ptr(X), _ptr(T), _ptr(W), _ptr(R0),
+                                                int(n_batches),
                                                 int(batch), lr, betas[0], betas[1], eps,
                                                 weight_decay, mean_row_weight,
                                                 losses.ctypes.data), "train_batches")
@@ -282,7 +314,8 @@ class LightSurrogate:
 
     def __init__(self, box: Box, op: OpticalParams, ro, *, tau_ns: float,
                  tau_rise_ns: float = 0.0, e_range=(10.0, 511.0), hidden=256, n_hidden_layers=3,
-                 seed=1, device=0, log_depth_mm: float = 0.0, fold: bool = True):
+                 seed=1, device=0, log_depth_mm: float = 0.0, fold: bool = True,
+                 direct: bool = True, images: bool = True):
         self.box, self.op = box, op
         self.ro = readout(ro) if isinstance(ro, SipmGrid) else ro
         self.ncell = self.ro.grid.nx * self.ro.grid.ny
@@ -308,6 +341,39 @@ class LightSurrogate:
         self.t_shift = self.tau_ns + self.tau_rise_ns
         self.t_scale = max(self.tau_ns + self.tau_rise_ns, 1.0)
         self.device = device
+        # the light that reaches the readout face straight from the deposit is computed, not
+        # learned (sharp near the face: it lands within d tan(theta_c) of the foot point)
+        self.direct = bool(direct) and not self.ro.disc and not self.ro.two_sided
+        # and, behind an air gap, its mirror images in the side walls near the readout face
+        self.images = self.direct and bool(images) and bool(op.air_gap)
+        cfg = SurrogateConfig(int(self.fold), int(self.direct), int(self.images), 0,
+                              self.log_depth_mm, self.t_shift, self.t_scale)
+        h = _bind().crysp_surrogate_create(self.net._h, C.byref(box), C.byref(op),
+                                           C.c_void_p(C.addressof(self.ro)), C.byref(cfg))
+        if not h:
+            raise _lib.CryspError("crysp_surrogate_create: " + _bind().crysp_mlp_last_error().decode())
+        self._sh = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_sh", None):
+                _bind().crysp_surrogate_destroy(self._sh)
+                self._sh = None
+        except Exception:
+            pass
+
+    def direct_light(self, xyz, out=None):
+        """The computed direct-light part D [n, ncell] (cell order i + nx j) at positions
+        xyz [n, 3] as given (no folding); a device tensor `out` receives it in place."""
+        xyz = xyz if hasattr(xyz, "data_ptr") else np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        n = int(xyz.shape[0])
+        _lib.check_array(xyz, "xyz", np.float32, (n, 3), device=self.device)
+        D = np.zeros((n, self.ncell), np.float32) if out is None else out
+        _lib.check_array(D, "D", np.float32, (n, self.ncell), device=self.device)
+        _lib.ready(xyz, D)
+        _mcheck(_bind().crysp_surrogate_direct_light(self._sh, _ptr(xyz), n, _ptr(D)),
+                "crysp_surrogate_direct_light")
+        return D
 
     def _depth(self, z):
         return -np.log(np.float32(self.box.L[2]) - z + np.float32(self.log_depth_mm))
@@ -325,13 +391,24 @@ class LightSurrogate:
 
     # ------------------------------------------------------------------ data ----
     def generate(self, n_samples: int, photons_per_sample: int, *, seed: int, pid0: int = 0,
+                 near_readout_frac: float = 0.0, near_readout_mm: float = 2.0,
                  ctx=None) -> TrainingSet:
         """Positions uniform in the crystal and energies uniform in e_range (Philox
-        domain 2), each traced at photons_per_sample photons."""
+        domain 2), each traced at photons_per_sample photons.  near_readout_frac of the
+        samples are drawn uniformly in the slab within near_readout_mm of the readout face
+        instead, where the maps change fastest."""
         from .events import synthetic_deposits
-        d = synthetic_deposits(self.box, n_events=n_samples, seed=seed, e_kev=self.e_range,
-                               n_fixed=photons_per_sample)
+        n_near = int(round(near_readout_frac * n_samples))
+        d = synthetic_deposits(self.box, n_events=n_samples - n_near, seed=seed,
+                               e_kev=self.e_range, n_fixed=photons_per_sample)
         X = np.column_stack([d.xyzt[:, :3], d.e_kev]).astype(np.float32)
+        if n_near:
+            L = self.box.L
+            dn = synthetic_deposits(self.box, n_events=n_near, seed=seed + 1, e_kev=self.e_range,
+                                    n_fixed=photons_per_sample,
+                                    lo=(0.0, 0.0, max(0.0, L[2] - near_readout_mm)),
+                                    hi=(L[0], L[1], L[2]))
+            X = np.concatenate([X, np.column_stack([dn.xyzt[:, :3], dn.e_kev]).astype(np.float32)])
         counts, tsum = mc_light_maps(self.box, self.op, self.ro, X[:, :3], photons_per_sample,
                                      seed=seed, tau_ns=self.tau_ns, tau_rise_ns=self.tau_rise_ns,
                                      pid0=pid0, ctx=ctx)
@@ -340,13 +417,17 @@ class LightSurrogate:
     def targets(self, ts: TrainingSet, device=None):
         """Observed per-photon rates (Poisson outputs, weight 1 / mean rate so gradients
         read as relative errors) and standardized mean times (weight c / (c + 20)), as
-        float32 [n, 2 * ncell] tensors on the device (numpy arrays with device=None)."""
+        float32 [n, 2 * ncell] tensors on the device (numpy arrays with device=None), and
+        the direct-light rates R0 [n, ncell] at the same (folded) positions (None without
+        the direct term): the network learns the rest of each rate."""
         import torch
         dev = torch.device("cpu") if device is None else device
         cnt = torch.from_numpy(np.ascontiguousarray(ts.counts)).to(dev)
         tsum = torch.from_numpy(np.ascontiguousarray(ts.tsum)).to(dev)
+        xyz_f = np.ascontiguousarray(ts.X[:, :3], np.float32).copy()
         if self.fold:
-            _, _, code = fold_xy(ts.X[:, 0], ts.X[:, 1], float(self.box.L[0]))
+            xc, yc, code = fold_xy(ts.X[:, 0], ts.X[:, 1], float(self.box.L[0]))
+            xyz_f[:, 0], xyz_f[:, 1] = xc, yc
             idx = torch.from_numpy(self.perms).to(dev)[torch.from_numpy(code.astype(np.int64)).to(dev)]
             cnt, tsum = torch.gather(cnt, 1, idx), torch.gather(tsum, 1, idx)
         p = cnt.double() / float(ts.nph)
@@ -360,15 +441,25 @@ class LightSurrogate:
         W[:, :self.ncell] = self.pois_w
         c = cnt.float()
         W[:, self.ncell:] = 0.25 * c / (c + 20.0)
-        return (T.numpy(), W.numpy()) if device is None else (T, W)
+        R0 = None
+        if self.direct:
+            if device is None:
+                R0 = self.direct_light(xyz_f)
+            else:
+                R0 = torch.empty((n, self.ncell), dtype=torch.float32, device=dev)
+                self.direct_light(torch.from_numpy(xyz_f).to(dev), out=R0)
+        if device is None:
+            return T.numpy(), W.numpy(), R0
+        return T, W, R0
 
-    def init_output_bias(self, T):
-        """Start the log-rate outputs at the log of each cell's mean rate (T: targets,
-        numpy or a tensor)."""
+    def init_output_bias(self, T, R0=None):
+        """Start the log-rate outputs at the log of each cell's mean learned rate (T:
+        targets, R0: the direct-light rates or None; numpy or tensors)."""
         par = self.net.get_params()
         n_out = 2 * self.ncell
         b = par.size - n_out
-        mean = T[:, :self.ncell].mean(0)
+        rest = T[:, :self.ncell] if R0 is None else T[:, :self.ncell] - R0
+        mean = rest.mean(0)
         mean = mean.cpu().numpy() if hasattr(mean, "cpu") else np.asarray(mean)
         par[b:b + self.ncell] = np.log(np.maximum(mean, 1e-9))
         par[b + self.ncell:] = 0.0
@@ -384,9 +475,9 @@ class LightSurrogate:
         if n == 0:
             raise ValueError("training set smaller than one batch")
         dev = torch.device("cuda", self.device)
-        T_d, W_d = self.targets(ts, device=dev)      # prepared on the device
+        T_d, W_d, R_d = self.targets(ts, device=dev)  # prepared on the device
         if self.net.get_params()[-2 * self.ncell:].any() == 0:
-            self.init_output_bias(T_d)
+            self.init_output_bias(T_d, R_d)
         X_d = torch.from_numpy(self.features(ts.X[:, :3], ts.X[:, 3])).to(dev)
         mrw = float(W_d.sum(1).mean() / W_d.shape[1])
         g = torch.Generator(device=dev).manual_seed(seed)     # permutations drawn on the device
@@ -395,6 +486,7 @@ class LightSurrogate:
         Xb = torch.empty((n, X_d.shape[1]), dtype=X_d.dtype, device=dev)
         Tb = torch.empty((n, T_d.shape[1]), dtype=T_d.dtype, device=dev)
         Wb = torch.empty((n, W_d.shape[1]), dtype=W_d.dtype, device=dev)
+        Rb = None if R_d is None else torch.empty((n, R_d.shape[1]), dtype=R_d.dtype, device=dev)
         losses = []
         for ep in range(epochs):
             lr_ep = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
@@ -402,9 +494,11 @@ class LightSurrogate:
             torch.index_select(X_d, 0, perm, out=Xb)
             torch.index_select(T_d, 0, perm, out=Tb)
             torch.index_select(W_d, 0, perm, out=Wb)
+            if Rb is not None:
+                torch.index_select(R_d, 0, perm, out=Rb)
             torch.cuda.synchronize()
             l = self.net.train_batches(Xb, Tb, Wb, n // batch, batch, lr=lr_ep,
-                                       mean_row_weight=mrw)
+                                       mean_row_weight=mrw, R0=Rb)
             losses.append(l)
             if verbose:
                 print(f"epoch {ep}: lr {lr_ep:.2e} loss {l.mean():.5f}", flush=True)
@@ -412,17 +506,45 @@ class LightSurrogate:
 
     # -------------------------------------------------------------- inference ----
     def predict(self, xyz, e_kev):
-        """Per-photon detection maps p [n, 64] and mean arrival times [n, 64] (ns after
-        the deposit)."""
-        Y = self.net.forward(self.features(xyz, e_kev))
-        p = np.exp(Y[:, :self.ncell])
-        t = Y[:, self.ncell:] * self.t_scale + self.t_shift
-        if self.fold:
-            xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
-            _, _, code = fold_xy(xyz[:, 0], xyz[:, 1], float(self.box.L[0]))
-            p = from_folded(p, code, self.perms)
-            t = from_folded(t, code, self.perms)
+        """Per-photon detection maps p [n, ncell] and mean arrival times [n, ncell] (ns
+        after the deposit) in actual cell order: features, direct light, network and
+        unfolding on the device."""
+        xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        dep = np.zeros((n, 4), np.float32)
+        dep[:, :3] = xyz
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(e_kev, np.float32), (n,)))
+        p = np.zeros((n, self.ncell), np.float32)
+        t = np.zeros((n, self.ncell), np.float32)
+        _mcheck(_bind().crysp_surrogate_predict(self._sh, dep.ctypes.data, e.ctypes.data, n,
+                                                p.ctypes.data, t.ctypes.data),
+                "crysp_surrogate_predict")
         return p, t
+
+    def sample(self, dep: Deposits, ev_first, use, yield_per_mev: float, *, seed: int,
+               out=None):
+        """Poisson event maps from the network for the deposits with use != 0 (all on the
+        device): returns (maps [nev, ncell] int32, mean times [nev, ncell])."""
+        nev = dep.n_events
+        ndep = int(dep.xyzt.shape[0])
+        if out is None:
+            maps = np.zeros((nev, self.ncell), np.int32)
+            tmean = np.zeros((nev, self.ncell), np.float32)
+        else:
+            maps, tmean = out
+        use8 = np.ascontiguousarray(use, np.uint8) if isinstance(use, np.ndarray) else use
+        _lib.check_array(dep.xyzt, "xyzt", np.float32, (ndep, 4), device=self.device)
+        _lib.check_array(dep.e_kev, "e_kev", np.float32, (ndep,), device=self.device)
+        _lib.check_array(use8, "use", np.uint8, (ndep,), device=self.device)
+        _lib.check_array(ev_first, "ev_first", np.int64, (nev + 1,), device=self.device)
+        _lib.check_array(maps, "maps", np.int32, numel=nev * self.ncell, device=self.device)
+        _lib.check_array(tmean, "tmean", np.float32, numel=nev * self.ncell, device=self.device)
+        _lib.ready(dep.xyzt, dep.e_kev, use8, ev_first, maps, tmean)
+        _mcheck(_bind().crysp_surrogate_sample(self._sh, _ptr(dep.xyzt), _ptr(dep.e_kev), _ptr(use8),
+                                               _ptr(ev_first), ndep, nev, float(yield_per_mev),
+                                               int(seed), _ptr(maps), _ptr(tmean), None),
+                "crysp_surrogate_sample")
+        return maps, tmean
 
     def bake(self, shape=(48, 48, 40), e_kev=None):
         """The network on a regular (gx, gy, gz) grid spanning the crystal, at one energy

@@ -119,6 +119,31 @@ def transport_accumulate(box: Box, op: OpticalParams, ro: Readout, tb: TimeBinni
     return acc
 
 
+def autotune_accumulate(box: Box, op: OpticalParams, ro, tb: TimeBinning, *, n_photons: int,
+                        seed: int, pos=None, positions=None, t0s=None, tau_ns: float = 0.0,
+                        tau_rise_ns: float = 0.0, max_bounces: int = 100_000, ctx=None,
+                        wg_sizes=(128, 256, 512), ppts=(1, 2, 4, 8), chunks=(32, 64, 128, 256),
+                        refill_mins=(6, 12, 24), reps: int = 2):
+    """Sweep every launch knob (workgroup size, photons per thread, queue chunk, refill
+    batch) on this point-source run (accumulate mode) and keep the fastest in the context.
+    Returns (best (wg, ppt, chunk, refill_min), times_ms[n_wg, n_ppt, n_chunk, n_refill])."""
+    ctx = ctx or _lib.default_context()
+    ro = _as_readout(ro)
+    px, py, pz, pt = _positions(pos, positions, t0s)
+    lists = [np.ascontiguousarray(v, np.int32) for v in (wg_sizes, ppts, chunks, refill_mins)]
+    times = np.zeros(tuple(v.size for v in lists), np.float32)
+    best = np.zeros(4, np.int32)
+    em = Emission(tau_ns, tau_rise_ns)
+    rc = _lib.load().crysp_autotune_accumulate(
+        ctx.handle, C.byref(box), C.byref(op), C.byref(ro), C.byref(tb), px.ctypes.data,
+        py.ctypes.data, pz.ctypes.data, None if pt is None else pt.ctypes.data, len(px),
+        int(n_photons), int(seed), int(max_bounces), C.byref(em),
+        *[a for v in lists for a in (v.ctypes.data, v.size)], int(reps), times.ctypes.data,
+        best.ctypes.data)
+    _lib.check(rc, "crysp_autotune_accumulate")
+    return tuple(int(v) for v in best), times
+
+
 def run_photons_ka(box: Box, op: OpticalParams, ro, tb: TimeBinning, *, n_photons: int,
                    seed: int, pos=None, positions=None, t0s=None, pid_offset: int = 0,
                    tau_ns: float, tau_rise_ns: float = 0.0, max_bounces: int = 100_000,

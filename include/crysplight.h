@@ -215,6 +215,32 @@ int crysp_autotune_events(crysp_ctx *ctx, const crysp_box *box,
                           const int32_t *ppts, int32_t n_ppt, int32_t reps,
                           float *times_ms, int32_t *best_wg, int32_t *best_ppt);
 
+/* Autotuners over every launch knob (no reference counterpart): each combination of
+ * wg_sizes x ppts x chunks x refill_mins (crysp_set_schedule, crysp_set_queue) is timed
+ * `reps` times on the given work (event mode, or accumulate mode for a point source), the
+ * best-of-reps kernel time [ms] of combination (a, b, q, r) is written to
+ * times_ms[((a * n_ppt + b) * n_chunk + q) * n_refill + r], and the fastest becomes the
+ * context's schedule and queue setting (best4 = wg, ppt, chunk, refill_min).  Results do not
+ * depend on the knobs. */
+int crysp_autotune_events_queue(crysp_ctx *ctx, const crysp_box *box,
+                                const crysp_optical_params *op, const crysp_readout *ro,
+                                const float *dep_xyzt, const int32_t *dep_nph,
+                                const int32_t *dep_event, int64_t ndep, int64_t nev,
+                                uint64_t seed, int32_t max_bounces, const crysp_emission *em,
+                                const int32_t *wg_sizes, int32_t n_wg, const int32_t *ppts,
+                                int32_t n_ppt, const int32_t *chunks, int32_t n_chunk,
+                                const int32_t *refill_mins, int32_t n_refill, int32_t reps,
+                                float *times_ms, int32_t *best4);
+int crysp_autotune_accumulate(crysp_ctx *ctx, const crysp_box *box,
+                              const crysp_optical_params *op, const crysp_readout *ro,
+                              const crysp_time_binning *tb, const float *px, const float *py,
+                              const float *pz, const float *pt, int32_t npos, int64_t n_photons,
+                              uint64_t seed, int32_t max_bounces, const crysp_emission *em,
+                              const int32_t *wg_sizes, int32_t n_wg, const int32_t *ppts,
+                              int32_t n_ppt, const int32_t *chunks, int32_t n_chunk,
+                              const int32_t *refill_mins, int32_t n_refill, int32_t reps,
+                              float *times_ms, int32_t *best4);
+
 /* Debug: the first n_draws raw u32 words of the streams pid0+1 .. pid0+n_pids,
  * produced on the device by the same Philox code the kernels use
  * (src/philox.jl:43-57).  out: n_pids * n_draws words, pid-major. */
@@ -322,18 +348,22 @@ int crysp_mlp_set_poisson_outputs(crysp_mlp *mlp, int32_t n_pois);
 /* Y [M x n_out] = MLP(X [M x n_in]); host or device pointers */
 int crysp_mlp_forward(crysp_mlp *mlp, const float *X, int64_t M, float *Y);
 /* Gradient of the loss (normalized by sum W) over one batch (W may be NULL: all ones),
- * without an update: grads [num_params] and loss (host pointers, optional). */
+ * without an update: grads [num_params] and loss (host pointers, optional).  R0 (may be
+ * NULL) [M x n_pois]: a known part of each Poisson rate, so that the rate is e^y + R0 and
+ * the loss w (rate - t log rate). */
 int crysp_mlp_gradient(crysp_mlp *mlp, const float *X, const float *T, const float *W,
-                       int32_t M, float *grads, float *loss);
-/* n_batches Adam steps on consecutive device-resident rows (batch a multiple of 64);
+                       const float *R0, int32_t M, float *grads, float *loss);
+/* n_batches Adam steps on consecutive device-resident rows (batch a multiple of 64) of
+ * X, T, W and R0 (rate offsets as in crysp_mlp_gradient, rows of n_pois; may be NULL);
  * mean_row_weight = mean over rows of sum(W) / n_out, the loss normalization.
  * losses[b] (host, optional) receives each step's loss.  A step is four kernels; up to 64
  * steps are captured in one hipGraph, which is kept for later calls with the same X, T, W
  * pointers, batch, optimizer settings and loss layout (the learning rate may change: it is
  * read from device memory).  Refill the same buffers between epochs to reuse it. */
 int crysp_mlp_train_batches(crysp_mlp *mlp, const float *X, const float *T, const float *W,
-                            int64_t n_batches, int32_t batch, float lr, float beta1, float beta2,
-                            float eps, float weight_decay, float mean_row_weight, float *losses);
+                            const float *R0, int64_t n_batches, int32_t batch, float lr,
+                            float beta1, float beta2, float eps, float weight_decay,
+                            float mean_row_weight, float *losses);
 
 /* Events from a surrogate baked on a regular grid: grid [gz][gy][gx][2][ncell] holds per
  * grid point the per-photon detection probability of every cell and the mean arrival
@@ -347,6 +377,49 @@ int crysp_surrogate_sample_events(int device, const float *grid, int32_t gx, int
                                   const uint8_t *use, const int64_t *ev_first, int64_t ndep,
                                   int64_t nev, double yield_per_mev, uint64_t seed,
                                   int32_t *maps, float *tmean, float *lam);
+
+/* A light-response surrogate of one crystal/optics set: a trained network (4 inputs,
+ * 2 * nx * ny outputs: per-photon log-rates of the cells, then standardized mean arrival
+ * times) plus, when cfg.direct, the light that reaches the readout face straight from the
+ * deposit, computed, not learned: D_c(x) = PDE / (4 pi) * integral over the directions
+ * whose straight path lands in cell c of T(cos theta) exp(-path / Lambda) (Fresnel
+ * transmission into the coupling, bulk attenuation over the path), weighted by a smoothstep
+ * of the depth Lz - z (1 within 2 mm of the readout face, 0 beyond 3 mm: the term is sharp
+ * only there).  The per-photon rate of cell c is e^{y_c} + D_c.  Inputs of the network: (x, y, depth feature, E) with x, y
+ * folded into 0 <= x <= y <= L/2 when cfg.fold (square face and grid; outputs in folded
+ * cell order), depth feature -log(Lz - z + log_depth_mm) when log_depth_mm > 0, else z.
+ * Mean times: y * t_scale + t_shift [ns after the deposit].  The surrogate keeps a
+ * reference to `mlp`, which must outlive it.  No reference counterpart. */
+typedef struct {
+    uint8_t fold;
+    uint8_t direct;
+    uint8_t images;          /* with direct and an air-gap wrap: add the direct light's
+                                mirror images in the side walls (total internal reflection)
+                                within 3 mm of a side wall and of the readout face */
+    uint8_t pad_;
+    float log_depth_mm;
+    float t_shift, t_scale;
+} crysp_surrogate_config;
+typedef struct crysp_surrogate crysp_surrogate;
+
+crysp_surrogate *crysp_surrogate_create(crysp_mlp *mlp, const crysp_box *box,
+                                        const crysp_optical_params *op, const crysp_readout *ro,
+                                        const crysp_surrogate_config *cfg);
+void crysp_surrogate_destroy(crysp_surrogate *s);
+/* D [n x nx*ny] (standard cell order i + nx j) at points xyz [n x 3] as given (no folding). */
+int crysp_surrogate_direct_light(crysp_surrogate *s, const float *xyz, int64_t n, float *D);
+/* Per-photon maps p [n x nx*ny] and mean arrival times t (optional) in actual cell order for
+ * deposits dep_xyzt [n x 4] with energies e_kev [n]. */
+int crysp_surrogate_predict(crysp_surrogate *s, const float *dep_xyzt, const float *e_kev,
+                            int64_t n, float *p, float *t);
+/* Events from the network, all on the device: for event e with deposits
+ * [ev_first[e], ev_first[e+1]) and use[d] != 0, lambda_c = sum_d yield * E_d / 1000 * p_c(x_d)
+ * and maps[e][c] ~ Poisson(lambda_c) (src/events.jl:11-22 algorithm, Philox domain 4);
+ * tmean/lam (optional) as crysp_surrogate_sample_events. */
+int crysp_surrogate_sample(crysp_surrogate *s, const float *dep_xyzt, const float *dep_e_kev,
+                           const uint8_t *use, const int64_t *ev_first, int64_t ndep, int64_t nev,
+                           double yield_per_mev, uint64_t seed, int32_t *maps, float *tmean,
+                           float *lam);
 
 /* Debug: C [M x N] = A * B in bf16 on the GEMM kernel (A [M][K], or [K][M] when a_t;
  * B [K][N], or [N][K] when b_t), for the MFMA lane-layout check. */

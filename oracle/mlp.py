@@ -71,9 +71,10 @@ def forward(p, X, shapes, shift=None, scale=None, bf16=False):
     return y, cache
 
 
-def loss_and_grad(p, X, T, Wt, shapes, shift=None, scale=None, bf16=False, n_pois=0):
-    """Loss normalized by sum(W): Poisson NLL w (e^y - t y) on the first n_pois outputs,
-    w (y - t)^2 on the rest; and its gradient."""
+def loss_and_grad(p, X, T, Wt, shapes, shift=None, scale=None, bf16=False, n_pois=0, R0=None):
+    """Loss normalized by sum(W): Poisson NLL w (r - t log r) of the rate r = e^y (+ R0 when
+    rate offsets are given) on the first n_pois outputs, w (y - t)^2 on the rest; and its
+    gradient."""
     params = unflatten(np.asarray(p, np.float64), shapes)
     y, cache = forward(p, X, shapes, shift, scale, bf16)
     w = np.ones_like(y) if Wt is None else np.asarray(Wt, np.float64)
@@ -84,8 +85,13 @@ def loss_and_grad(p, X, T, Wt, shapes, shift=None, scale=None, bf16=False, n_poi
     gmat = 2.0 * w * d
     if n_pois:
         e = np.exp(y[:, :n_pois])
-        lmat[:, :n_pois] = w[:, :n_pois] * (e - t[:, :n_pois] * y[:, :n_pois])
-        gmat[:, :n_pois] = w[:, :n_pois] * (e - t[:, :n_pois])
+        if R0 is None:
+            lmat[:, :n_pois] = w[:, :n_pois] * (e - t[:, :n_pois] * y[:, :n_pois])
+            gmat[:, :n_pois] = w[:, :n_pois] * (e - t[:, :n_pois])
+        else:
+            r = e + np.asarray(R0, np.float64)
+            lmat[:, :n_pois] = w[:, :n_pois] * (r - t[:, :n_pois] * np.log(r))
+            gmat[:, :n_pois] = w[:, :n_pois] * (1.0 - t[:, :n_pois] / r) * e
     L = float(lmat.sum() / ws)
     rnd = round_bf16 if bf16 else (lambda a: np.asarray(a, np.float64))
     dz = gmat / ws

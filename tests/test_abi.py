@@ -232,3 +232,14 @@ def test_check_array_torch_rules():
         _lib.check_array(torch.zeros((4, 4))[:, :2], "x", np.float32)
     with pytest.raises(ValueError, match="dtype"):
         _lib.check_array(torch.zeros(3, dtype=torch.int64), "n", np.int32)
+
+
+def test_every_pointer_taking_entry_point_has_a_signature():
+    """ctypes passes an undeclared pointer argument as a 32-bit int: every entry point the
+    Python host calls with a context handle is declared in _lib.SIGNATURES."""
+    import pathlib
+    called = set()
+    for f in pathlib.Path(_lib.HERE).glob("*.py"):
+        called |= set(re.findall(r"load\(\)\.(crysp_\w+)\(", f.read_text()))
+    missing = sorted(n for n in called if n not in _lib.SIGNATURES)
+    assert not missing, missing

@@ -221,6 +221,38 @@ def test_events_schedule_independent_and_autotune(ctx, bgo_events):
         ctx.set_schedule(*old)
 
 
+def test_full_knob_autotuners(ctx, bgo_events):
+    """The tuners over workgroup size, photons per thread, queue chunk and refill batch (event
+    and accumulate mode) time every combination, keep the fastest in the context, and the
+    maps computed with the tuned knobs are still the oracle's."""
+    from crysplight_amd import transport as T
+    dep, _ = bgo_events
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    old = ctx.get_schedule()
+    try:
+        best, times = E.autotune_queue(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                                       wg_sizes=(128, 256), ppts=(4,), chunks=(32, 128),
+                                       refill_mins=(6, 12), reps=1)
+        assert times.shape == (2, 1, 2, 2) and np.all(np.isfinite(times)) and np.all(times > 0)
+        assert best[0] in (128, 256) and best[1] == 4 and best[2] in (32, 128) and best[3] in (6, 12)
+        assert ctx.get_schedule() == best[:2]
+        res = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+        _check(res, ref, dep.n_events)
+        best_a, times_a = T.autotune_accumulate(K.C1_BOX, K.C1_OP, K.C1_RO, K.C1_TB, n_photons=100_000,
+                                                seed=5, pos=K.C1_POS, tau_ns=K.C1_TAU, ctx=ctx,
+                                                wg_sizes=(256,), ppts=(1, 4), chunks=(64, 128),
+                                                refill_mins=(12,), reps=1)
+        assert times_a.shape == (1, 2, 2, 1) and np.all(times_a > 0)
+        acc = T.transport_accumulate(K.C1_BOX, K.C1_OP, K.C1_RO, K.C1_TB, n_photons=100_000, seed=5,
+                                     pos=K.C1_POS, tau_ns=K.C1_TAU, ctx=ctx)
+        r = oracle.run_photons(K.C1_BOX, K.C1_OP, K.C1_RO, K.C1_TB, n_photons=100_000,
+                               seed=5, pos=K.C1_POS, tau_ns=K.C1_TAU)
+        assert np.array_equal(acc.counts, r["counts"]) and acc.ndet == r["ndet"]
+    finally:
+        ctx.set_schedule(*old)
+        ctx.set_queue(128, 12)
+
+
 def test_events_queue_parameters_do_not_change_results(ctx, bgo_events):
     """The work-queue grab size and the lane-refill batch only reorder which lane runs which
     photon: every (chunk, refill_min) gives the oracle's maps, including a chunk of one

@@ -19,8 +19,6 @@ RO8 = readout(SipmGrid(3.2, 8, 8))
 def _noalloc(ctx, dep, xyzt, nph, ev, out, seed, tau, pid0=0):
     lib = _lib.load()
     f = lib.crysp_run_events_noalloc
-    f.restype = C.c_int
-    f.argtypes = lib.crysp_run_events.argtypes
     maps, tmin, npe = out
     ntot = C.c_int64()
     em = Emission(tau, 0.0)
@@ -69,8 +67,6 @@ def test_noalloc_refuses_oversize_and_host_arrays(ctx):
     (xyzt, nph, ev), out = _tensors(dep, dev)
     lib = _lib.load()
     f = lib.crysp_run_events_noalloc
-    f.restype = C.c_int
-    f.argtypes = lib.crysp_run_events.argtypes
     em = Emission(K.C2_TAU, 0.0)
     maps, tmin, npe = out
     # more deposits than the scratch array holds: refused before any launch
