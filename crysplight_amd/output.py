@@ -7,8 +7,11 @@ write_events       <- the events.h5 writer of scripts/shoot_gammas.jl:12-17,60-9
 Both files are HDF5 (written by crysplight_amd.h5) under the reference's names and
 dtypes, with the arrays in the order h5py presents the reference's Julia (column-major)
 files: light counts as (nbins+1, ny, nx), first_time_ns (ny, nx); events maps/tmin
-(n, ny, nx) with maps narrowed to UInt16 as in the reference, xyz (n, 3).  Root
-attributes carry the reference's keys.
+(n, ny, nx), xyz (n, 3).  Root attributes carry the reference's keys.  The event
+datasets are chunked along the event axis, 4096 events per chunk with the full other
+dimensions (scripts/shoot_gammas.jl:11-17).  maps are narrowed to UInt16 modulo 2^16:
+the reference counts each photoelectron with `maps[ii, jj, ev] += UInt16(1)`
+(src/events.jl:49,81), and Julia's UInt16 addition wraps.
 """
 from __future__ import annotations
 
@@ -61,6 +64,17 @@ def write_output(st, acc, elapsed: float, write: bool = True, verbose: bool = Tr
     return summary
 
 
+EVENT_CHUNK = 4096
+
+
+def uint16_maps(maps) -> np.ndarray:
+    """Per-event SiPM counts as the reference's UInt16 matrix holds them: modulo 2^16."""
+    maps = np.asarray(maps)
+    if maps.dtype.kind not in "iu" or (maps.dtype.kind == "i" and maps.size and maps.min() < 0):
+        raise ValueError("maps must hold non-negative integer counts")
+    return (maps.astype(np.uint64) & 0xFFFF).astype(np.uint16)
+
+
 def write_events(c, r: dict, elapsed: float, outdir, write: bool = True, verbose: bool = True) -> dict:
     nev = c.n_events
     maps = np.asarray(r["maps"])
@@ -74,11 +88,11 @@ def write_events(c, r: dict, elapsed: float, outdir, write: bool = True, verbose
                  "tmin_nohit": "Inf"}
         h5.write(outdir / "events.h5", {
             "edep_kev": np.asarray(r["edep"], np.float32), "npe": np.asarray(r["npe"], np.int32),
-            "maps": np.minimum(maps, 65535).astype(np.uint16), "tmin_ns": np.asarray(r["tmin"], np.float32),
+            "maps": uint16_maps(maps), "tmin_ns": np.asarray(r["tmin"], np.float32),
             "xyz1_mm": np.asarray(r["xyz1"], np.float32), "e1_kev": np.asarray(r["e1"], np.float32),
             "xyz2_mm": np.asarray(r["xyz2"], np.float32), "e2_kev": np.asarray(r["e2"], np.float32),
             "er_kev": np.asarray(r["er"], np.float32), "int_type": np.asarray(r["int_type"], np.int8),
-            "n_int": np.asarray(r["n_int"], np.int16)}, attrs)
+            "n_int": np.asarray(r["n_int"], np.int16)}, attrs, chunk_events=EVENT_CHUNK)
     edep = np.asarray(r["edep"])
     npe = np.asarray(r["npe"])
     inter = edep > 0

@@ -30,6 +30,11 @@ with h5py.File(sys.argv[1], "r") as f:
     for k in f:
         a = f[k][()]
         out["datasets"][k] = [a.dtype.str, list(a.shape), hashlib.sha256(a.tobytes()).hexdigest()]
+        out.setdefault("chunks", {})[k] = list(f[k].chunks) if f[k].chunks else None
+        if f[k].chunks and a.shape[0] > 1:     # partial reads through the chunk index
+            lo, hi = a.shape[0] // 3, a.shape[0] - 1
+            part = f[k][lo:hi]
+            out.setdefault("slices", {})[k] = bool(np.array_equal(part, a[lo:hi]))
     for k, v in f.attrs.items():
         if isinstance(v, bytes):
             out["attrs"][k] = ["bytes", v.decode()]
@@ -63,9 +68,12 @@ def _sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def _check(path: Path, datasets: dict, attrs: dict):
+def _check(path: Path, datasets: dict, attrs: dict, chunks: dict | None = None):
     d = _dump(path)
     assert d["keys"] == sorted(datasets)
+    if chunks is not None:
+        assert d.get("chunks", {}) == {k: chunks.get(k) for k in datasets}
+        assert all(d.get("slices", {}).values())
     for k, a in datasets.items():
         a = np.asarray(a)
         assert d["datasets"][k] == [a.dtype.str, list(a.shape), _sha(a)], k
@@ -161,7 +169,7 @@ def test_events_h5_layout(tmp_path):
     write_events(c, r, 0.1, tmp_path / "out", verbose=False)
     p = tmp_path / "out" / "events.h5"
     want = {"edep_kev": r["edep"].astype(np.float32), "npe": r["npe"].astype(np.int32),
-            "maps": np.minimum(r["maps"], 65535).astype(np.uint16), "tmin_ns": r["tmin"].astype(np.float32),
+            "maps": (r["maps"] % 65536).astype(np.uint16), "tmin_ns": r["tmin"].astype(np.float32),
             "xyz1_mm": r["xyz1"].astype(np.float32), "e1_kev": r["e1"].astype(np.float32),
             "xyz2_mm": r["xyz2"].astype(np.float32), "e2_kev": r["e2"].astype(np.float32),
             "er_kev": r["er"].astype(np.float32), "int_type": r["int_type"].astype(np.int8),
@@ -170,7 +178,40 @@ def test_events_h5_layout(tmp_path):
              "backend": "hip-gfx950",
              "int_type_key": "-1 = no interaction, 0 = direct photoelectric, X>=1 = X Compton scatters",
              "tmin_nohit": "Inf"}
-    _check(p, want, attrs)
+    _check(p, want, attrs, chunks={k: [n] + list(a.shape[1:]) for k, a in want.items()})
+
+
+@needs_reader
+def test_chunked_datasets(tmp_path):
+    """Chunks of 4096 events along the first h5py axis with the full other dimensions,
+    the edge chunk partial, more chunks than the library's default B-tree node holds."""
+    rng = np.random.default_rng(8)
+    ds = {"maps": rng.integers(0, 65536, (4096 * 3 + 17, 8, 8)).astype(np.uint16),
+          "tmin_ns": rng.random((4096, 8, 8)).astype(np.float32),
+          "xyz1_mm": rng.random((4095, 3)).astype(np.float32),
+          "npe": rng.integers(0, 9000, 70 * 64 + 1).astype(np.int32),
+          "one": np.array([7], np.int16)}
+    p = tmp_path / "chunked.h5"
+    h5.write(p, ds, {"n_events": 1}, chunk_events=4096)
+    _check(p, ds, {"n_events": 1}, chunks={"maps": [4096, 8, 8], "tmin_ns": [4096, 8, 8],
+                                            "xyz1_mm": [4095, 3], "npe": [4096], "one": [1]})
+    q = tmp_path / "many_chunks.h5"
+    h5.write(q, {"npe": ds["npe"]}, chunk_events=64)       # 71 chunks: K raised above 32
+    _check(q, {"npe": ds["npe"]}, {}, chunks={"npe": [64]})
+
+
+def test_uint16_maps_wrap():
+    """src/events.jl:49,81: `maps[ii, jj, ev] += UInt16(1)` wraps at 2^16."""
+    from crysplight_amd.output import uint16_maps
+    m = np.array([0, 1, 65535, 65536, 65537, 200000], np.int64)
+    assert uint16_maps(m).tolist() == [0, 1, 65535, 0, 1, 200000 % 65536]
+    # counting one by one in UInt16 gives the same
+    acc = np.zeros(1, np.uint16)
+    for _ in range(65536 + 5):
+        acc += np.uint16(1)
+    assert acc[0] == uint16_maps(np.array([65536 + 5]))[0]
+    with pytest.raises(ValueError):
+        uint16_maps(np.array([-1]))
 
 
 @needs_reader
