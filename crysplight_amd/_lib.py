@@ -38,6 +38,7 @@ SIGNATURES = {
     "crysp_set_schedule": (C.c_int, [_vp, _i32, _i32]),
     "crysp_get_schedule": (C.c_int, [_vp, _vp, _vp]),
     "crysp_set_queue": (C.c_int, [_vp, _i32, _i32]),
+    "crysp_set_event_sessions": (C.c_int, [_vp, _i32, _i32]),
     "crysp_autotune_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
                                         _i32, _pem, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
     "crysp_autotune_events_queue": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64,
@@ -136,6 +137,12 @@ class Context:
 
     def set_queue(self, chunk: int, refill_min: int):
         check(load().crysp_set_queue(self._h, chunk, refill_min), "crysp_set_queue")
+
+    def set_event_sessions(self, enable: bool, piece: int = 512):
+        """Event mode: workgroups own whole events (one write per map row) when the
+        deposits are ordered by event, or per-detection atomics (enable False)."""
+        check(load().crysp_set_event_sessions(self._h, int(bool(enable)), int(piece)),
+              "crysp_set_event_sessions")
 
     def get_schedule(self) -> tuple[int, int]:
         a, b = C.c_int32(), C.c_int32()

@@ -60,6 +60,12 @@ struct KParams {
     int32_t refill_min;                       // refill lanes once this many are idle
     int32_t hist_words;                       // nx*ny*(nbins+1)
     int32_t nbox;                             // crystals of a several-crystal event queue
+    // event mode, events owned by workgroups
+    int32_t gate;                             // 0 run; 1 run if the deposits are ordered by
+                                              // event, 2 if not (device flag bit 8 decides)
+    int32_t piece;                            // photons per workgroup claim of an event
+    int32_t pool_cap;                         // partial maps the helpers may write
+    int64_t help_window;                      // helpers look among the last this many events
 };
 
 template <uint32_t OPT>
@@ -221,6 +227,19 @@ __device__ __forceinline__ double fma_k(double a, double b, double c)
     return r;
 }
 
+// z * b + c for binary64 constants b (a scalar register pair) and c, the first step of a
+// Horner chain: c is moved into VGPRs at the use instead of the leading coefficient being
+// held in a VGPR pair across the whole kernel.  The product is commutative, so this is the
+// same fma as fma_k(b, z, c).
+__device__ __forceinline__ double fma_first(double z, double b, double c)
+{
+    double cv;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(cv) : "s"(c));
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(z), "s"(b), "v"(cv));
+    return r;
+}
+
 __device__ __forceinline__ float f_log(float x)
 {
     const uint32_t b = __float_as_uint(x);
@@ -302,14 +321,12 @@ __device__ __forceinline__ void f_sincos_tilt(float af, float &sf, float &cf)
     if (!(af < 0.125f)) { f_sincos(af, sf, cf); return; }
     const double a = (double)af;
     const double z = a * a;
-    double ps = -2.505210838544172e-08;                      // -1/11!
-    ps = fma_k(ps, z, 2.7557319223985893e-06);       // 1/9!
+    double ps = fma_first(z, -2.505210838544172e-08, 2.7557319223985893e-06);  // -1/11!, 1/9!
     ps = fma_k(ps, z, -0.0001984126984126984);       // -1/7!
     ps = fma_k(ps, z, 0.008333333333333333);         // 1/5!
     ps = fma_k(ps, z, -0.16666666666666666);         // -1/3!
     const double sn = __builtin_fma(a * z, ps, a);
-    double pc = 2.08767569878681e-09;                        // 1/12!
-    pc = fma_k(pc, z, -2.755731922398589e-07);       // -1/10!
+    double pc = fma_first(z, 2.08767569878681e-09, -2.755731922398589e-07);  // 1/12!, -1/10!
     pc = fma_k(pc, z, 2.48015873015873e-05);         // 1/8!
     pc = fma_k(pc, z, -0.001388888888888889);        // -1/6!
     pc = fma_k(pc, z, 0.041666666666666664);         // 1/4!

@@ -44,9 +44,21 @@ struct KBuffers {
     unsigned long long *ev_tsum;                    // events, optional: sum of (t - t0) in 2^-10 ns
     const float4 *boxes;                            // events, several crystals: (Lx, Ly, Lz, -)
     const int32_t *dep_box;                         // events, several crystals: box of a deposit
-    unsigned long long *cursor;                     // work-queue cursor (zeroed)
+    unsigned long long *cursor;                     // work-queue cursor (zeroed); the event
+                                                    // cursor when workgroups own events
     unsigned long long *opcounts;                   // 10 op counters (COUNT only)
     unsigned int *flag;                             // events: input error flags
+    // events owned by workgroups
+    unsigned long long *ev_next;                    // per event: its next unclaimed photon
+    const unsigned long long *ev_end;               // per event: one past its last photon
+    uint32_t *ev_head;                              // per event: first partial map + 1, 0 none
+    uint32_t *pool_link;                            // per partial: the next one + 1, 0 none
+    uint32_t *pool_data;                            // per partial: nxy counts, nxy time keys
+    uint32_t *sess_ctr;                             // [0] partials used, [1] events written
+    uint32_t *open_ev;                              // per workgroup: the event it owns and has
+                                                    // not yet claimed to the end, or ~0
+    float *tmin_out;                                // tmin (f32), written by the event's owner
+    int32_t *npe_out;
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -80,9 +92,253 @@ __device__ __forceinline__ int64_t find_deposit(const int64_t *off, int64_t ndep
     return lo;
 }
 
+// ---- events owned by workgroups ---------------------------------------------------
+// With the deposits ordered by event, each event's photons form one contiguous range.  A
+// workgroup opens a session on an event (the next one from the event cursor), claims the
+// event's photons `piece` at a time from that event's own cursor and hands them to its
+// waves, which add detections to the session's map in LDS.  Once the event's cursor has
+// run out (the session is closed) and the last of its photons has ended, one wave writes
+// the event's map row, tmin and npe with plain stores: the event has one writer, so no
+// global atomic touches the maps.  When every event has an owner, workgroups left without
+// work help: they open sessions on events with photons still unclaimed, and their sums go
+// to partial maps that finish_sessions adds in afterwards.  Counts are sums and first
+// times minima, so the maps equal those of any other order of the same detections.
+constexpr int NSESS = 4;                            // session slots per workgroup
+constexpr int SESS_MAX_CELLS = 256;                 // SiPMs a session map holds
+constexpr uint32_t SS_CLOSED = 0x80000000u;         // the event's photons are all claimed
+constexpr uint32_t SS_FREE = 0x40000000u;           // the slot holds no session
+constexpr uint32_t SS_COUNT = 0x3FFFFFFFu;          // photons claimed and not yet ended
+constexpr uint32_t CUR_NONE = 0xFFu, CUR_DONE = 0xFEu;
+constexpr uint32_t FLAG_UNSORTED = 8u, FLAG_INTERNAL = 16u;
+// words of the session area in LDS: lock, current slot, the workgroup's claimed photons
+// [wnext, wend) of slot wslot, a spare word, then per slot its state, event and
+// partial (-1 for the event's owner), then per slot nxy counts and nxy time keys
+enum : int { SH_LOCK = 0, SH_CUR = 1, SH_WNEXT = 2, SH_WEND = 4, SH_WSLOT = 6, SH_SPARE = 7,
+             SH_STATE = 8, SH_EV = 8 + NSESS, SH_PART = 8 + 2 * NSESS, SH_MAPS = 32 };
+static_assert(SH_PART + NSESS <= SH_MAPS, "session header overlaps the maps");
+
+__host__ __device__ constexpr size_t sess_lds_words(int nxy)
+{
+    return (size_t)SH_MAPS + (size_t)NSESS * 2 * (size_t)nxy;
+}
+
+__device__ __forceinline__ uint32_t lds_acquire(uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS words the whole wave reads at one address, made wave-uniform (scalar registers)
+__device__ __forceinline__ uint32_t lds_uni(const uint32_t *p)
+{
+    return __builtin_amdgcn_readfirstlane(*p);
+}
+
+__device__ __forceinline__ uint64_t lds_u64(const uint32_t *p)
+{
+    return (uint64_t)lds_uni(p) | ((uint64_t)lds_uni(p + 1) << 32);
+}
+
+__device__ __forceinline__ void lds_set_u64(uint32_t *p, uint64_t v)
+{
+    p[0] = (uint32_t)v;
+    p[1] = (uint32_t)(v >> 32);
+}
+
+// A field of the transport kernel's arguments (KParams, then KBuffers, as the kernel
+// argument segment lays them out) read again at each use.  The session code's pointers and
+// sizes are used rarely; read this way they hold no scalar registers across the photon
+// loop, where the compiler would otherwise keep every one of them live.  Only for code
+// that runs inside transport_kernel.
+static_assert(sizeof(KParams) % alignof(KBuffers) == 0, "KBuffers follows KParams directly");
+template <typename T>
+__device__ __forceinline__ T karg_at(size_t off)
+{
+    const __attribute__((address_space(4))) char *kp =
+        (const __attribute__((address_space(4))) char *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    return *(const __attribute__((address_space(4))) T *)(kp + off);
+}
+#define KB(f) karg_at<decltype(KBuffers::f)>(sizeof(KParams) + offsetof(KBuffers, f))
+#define KP(f) karg_at<decltype(KParams::f)>(offsetof(KParams, f))
+
+// Session q's sums written out by one wave (every lane; q wave-uniform), the slot freed.
+__device__ void sess_flush(uint32_t *S, int q, int nxy, const KBuffers &B)
+{
+    const int64_t e = (int32_t)lds_uni(&S[SH_EV + q]);
+    const int32_t part = (int32_t)lds_uni(&S[SH_PART + q]);
+    uint32_t *m = S + SH_MAPS + q * 2 * nxy;
+    // destinations: the event's rows (owner) or the partial map (helper)
+    uint32_t *dc = part < 0 ? (uint32_t *)KB(maps) + e * nxy : KB(pool_data) + (int64_t)part * 2 * nxy;
+    uint32_t *dk = part < 0 ? (uint32_t *)KB(tmin_out) + e * nxy : dc + nxy;
+    uint32_t sum = 0u;
+#pragma unroll 1
+    for (int c = (int)__lane_id(); c < nxy; c += 64) {
+        const uint32_t cnt = m[c], key = m[nxy + c];
+        m[c] = 0u;
+        m[nxy + c] = KEY_INF;
+        dc[c] = cnt;
+        dk[c] = part < 0 ? __float_as_uint(key_time(key)) : key;
+        sum += cnt;
+    }
+#pragma unroll
+    for (int m2 = 32; m2 >= 1; m2 >>= 1) sum += __shfl_xor(sum, m2);
+    if (__lane_id() == 0) {
+        if (part < 0) {
+            KB(npe_out)[e] = (int32_t)sum;
+            atomicAdd(&KB(sess_ctr)[1], 1u);
+        } else if (sum) {
+            // the partial is read by finish_sessions, after this kernel: no fence needed
+            KB(pool_link)[part] = atomicExch(&KB(ev_head)[e], (uint32_t)part + 1u);
+        }
+        lds_release(&S[SH_STATE + q], SS_FREE);
+    }
+}
+
+// An event a helper can join: one that its owner has not yet claimed to the end (the
+// owners publish their current event) with at least a chunk of photons unclaimed.  The
+// scan starts past this workgroup's own entry, four entries per lane at a time; -1 when no
+// owner has such an event.  Wave-uniform.
+__device__ int64_t find_help(const KParams &P, const KBuffers &B)
+{
+    const uint32_t nw = gridDim.x;
+    const uint32_t *open = KB(open_ev);
+    for (uint32_t o = 0; o < nw; o += 256) {
+        uint32_t evs[4];
+        uint64_t m[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t k = o + (uint32_t)__lane_id() + 64u * i;
+            evs[i] = k < nw ? __hip_atomic_load(&open[(blockIdx.x + 1u + k) % nw], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : ~0u;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m[i] = __ballot(evs[i] < (uint32_t)KP(nev));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint64_t mm = m[i];
+            while (mm) {
+                const int l = (int)__builtin_ctzll(mm);
+                const int64_t e = (int64_t)__builtin_amdgcn_readlane(evs[i], l);
+                const unsigned long long nx =
+                    __hip_atomic_load(&KB(ev_next)[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (nx + (unsigned long long)KP(chunk) <= KB(ev_end)[e]) return e;
+                mm &= mm - 1;
+            }
+        }
+    }
+    return -1;
+}
+
+// A wave's next photons from its workgroup's sessions, one action per call: take up to a
+// chunk of the photons the workgroup has claimed ([b, e) of slot q); or claim the next
+// piece of the current session's event (closing the session when the event runs out); or
+// open a session on a new event (an owner) or on one with photons left (a helper); or
+// find there is no more work (done).  b == e when this call took no photons (also when
+// another wave holds the lock).  A session closed with no photon in flight is returned
+// in flush_q for the caller to write out.  Wave-uniform.
+__device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint64_t &b,
+                          uint64_t &e, int &q, bool &done, int &flush_q)
+{
+    const bool leader = __lane_id() == 0;
+    uint32_t locked = 0u;
+    if (leader) {
+        uint32_t expect = 0u;
+        locked = __hip_atomic_compare_exchange_strong(&S[SH_LOCK], &expect, 1u, __ATOMIC_ACQUIRE,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                     ? 1u : 0u;
+    }
+    if (!__builtin_amdgcn_readfirstlane(locked)) return;
+    const uint64_t wn = lds_u64(S + SH_WNEXT), we = lds_u64(S + SH_WEND);
+    const uint32_t cur = lds_uni(&S[SH_CUR]);
+    if (wn < we) {                                  // photons the workgroup has claimed
+        const uint64_t n = we - wn < (uint64_t)KP(chunk) ? we - wn : (uint64_t)KP(chunk);
+        b = wn;
+        e = wn + n;
+        q = (int)lds_uni(&S[SH_WSLOT]);
+        if (leader) lds_set_u64(S + SH_WNEXT, wn + n);
+    } else if (cur == CUR_DONE) {
+        done = true;
+    } else if (cur < (uint32_t)NSESS) {             // the next piece of the session's event
+        const int64_t ev = (int32_t)lds_uni(&S[SH_EV + cur]);
+        const bool helper = (int32_t)lds_uni(&S[SH_PART + cur]) >= 0;
+        // owners claim pieces; helpers, who share the event with its owner, a chunk at a time
+        const unsigned long long piece = (unsigned long long)(helper ? KP(chunk) : KP(piece));
+        unsigned long long g = 0;
+        if (leader) g = atomicAdd(&KB(ev_next)[ev], piece);
+        g = first_lane_u64(g);
+        const unsigned long long gend = KB(ev_end)[ev];
+        const unsigned long long n = g < gend ? (gend - g < piece ? gend - g : piece) : 0ull;
+        const bool close = g + piece >= gend;
+        uint32_t old = 0u;
+        if (leader)
+            old = __hip_atomic_fetch_add(&S[SH_STATE + cur], (uint32_t)n + (close ? SS_CLOSED : 0u),
+                                         __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        old = __builtin_amdgcn_readfirstlane(old);
+        if (leader) {
+            if (n) {
+                const uint64_t n2 = n < (uint64_t)KP(chunk) ? n : (uint64_t)KP(chunk);
+                lds_set_u64(S + SH_WNEXT, g + n2);
+                lds_set_u64(S + SH_WEND, g + n);
+                S[SH_WSLOT] = cur;
+            }
+            if (close) {
+                S[SH_CUR] = CUR_NONE;
+                if (!helper) KB(open_ev)[blockIdx.x] = ~0u;     // nothing left to help with
+            }
+        }
+        if (n) {                                    // the wave takes the piece's first chunk
+            b = g;
+            e = g + (n < (uint64_t)KP(chunk) ? n : (uint64_t)KP(chunk));
+            q = (int)cur;
+        }
+        if (close && (old & SS_COUNT) + (uint32_t)n == 0u) flush_q = (int)cur;
+    } else {                                        // open a session in a free slot
+        int fq = -1;
+        for (int s = 0; s < NSESS; ++s)
+            if (fq < 0 && __builtin_amdgcn_readfirstlane(lds_acquire(&S[SH_STATE + s])) == SS_FREE) fq = s;
+        if (fq >= 0) {
+            unsigned long long ec = 0;
+            if (leader) ec = atomicAdd(KB(cursor), 1ull);
+            ec = first_lane_u64(ec);
+            int64_t ev = (int64_t)ec;               // the event's owner
+            int32_t part = -1;
+            if (ec >= (unsigned long long)KP(nev)) {
+                ev = find_help(P, B);
+                if (ev >= 0) {
+                    uint32_t pi = 0u;
+                    if (leader) pi = atomicAdd(&KB(sess_ctr)[0], 1u);
+                    pi = __builtin_amdgcn_readfirstlane(pi);
+                    if (pi < (uint32_t)KP(pool_cap)) part = (int32_t)pi;
+                    else ev = -1;
+                }
+            }
+            if (leader) {
+                if (ev < 0) {
+                    S[SH_CUR] = CUR_DONE;
+                } else {
+                    S[SH_EV + fq] = (uint32_t)ev;
+                    S[SH_PART + fq] = (uint32_t)part;
+                    S[SH_STATE + fq] = 0u;
+                    S[SH_CUR] = (uint32_t)fq;
+                    if (part < 0) KB(open_ev)[blockIdx.x] = (uint32_t)ev;     // helpers may join
+                }
+            }
+            if (ev < 0) done = true;
+        }
+    }
+    if (leader) lds_release(&S[SH_LOCK], 0u);
+}
+
 // Seven waves per SIMD (at most 72 VGPRs): the waves cover the LDS reads of the Philox
 // rings and each other's dependent binary64 chains.
-template <int MODE, bool COUNT, bool MULTIBOX = false, uint32_t OPT = OPT_GENERIC>
+template <int MODE, bool COUNT, bool MULTIBOX = false, uint32_t OPT = OPT_GENERIC,
+          bool SESS = false>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 7)))
 transport_kernel(const KParams P, const KBuffers B)
 {
@@ -92,6 +348,23 @@ transport_kernel(const KParams P, const KBuffers B)
     uint32_t *lds = lds_dyn + 16u * blockDim.x;
     __shared__ unsigned long long lds_ctr[9];       // accumulate: the Accumulator counters
     const int nxy = P.nx * P.ny;
+    constexpr bool sess = MODE == M_EVENTS && SESS;
+    if (MODE == M_EVENTS && P.gate) {
+        // the device decides between the two event paths: ordered deposits or not
+        const bool unsorted =
+            (__hip_atomic_load(B.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_UNSORTED) != 0u;
+        if (unsorted != (P.gate == 2)) return;
+    }
+    if (sess) {
+        for (int w = threadIdx.x; w < (int)sess_lds_words(nxy); w += blockDim.x) {
+            uint32_t v = 0u;
+            if (w == SH_CUR) v = CUR_NONE;
+            else if (w >= SH_STATE && w < SH_STATE + NSESS) v = SS_FREE;
+            else if (w >= SH_MAPS && ((w - SH_MAPS) / nxy) % 2 == 1) v = KEY_INF;
+            lds[w] = v;
+        }
+        __syncthreads();
+    }
     if (MODE == M_ACCUM) {
         for (int w = threadIdx.x; w < P.hist_words + nxy; w += blockDim.x)
             lds[w] = w < P.hist_words ? 0u : KEY_INF;
@@ -120,6 +393,11 @@ transport_kernel(const KParams P, const KBuffers B)
 
     uint64_t pool_next = 0, pool_end = 0;           // wave-uniform work pool
     int64_t pool_dep = 0;                           // deposit holding the pool chunk's start
+    int pool_slot = 0;                              // sessions: the pool's session slot
+    uint32_t flush_pending = 0u;                    // sessions: slots to write out (bits)
+    uint64_t unreported = 0;                        // sessions: lanes whose photon ended and
+                                                    // is not yet taken off its session's count
+    uint32_t idle_iters = 0;                        // sessions: iterations of an idle wait
     bool exhausted = false;
     // event mode reads the photon total the offsets scan produced
     const uint64_t N = (MODE == M_EVENTS && P.n_photons < 0) ? (uint64_t)B.dep_off[P.ndep]
@@ -130,12 +408,48 @@ transport_kernel(const KParams P, const KBuffers B)
         // ---- refill idle lanes, in batches of at least refill_min ------------------
         const uint64_t need = __ballot(!alive);
         const uint32_t nneed = (uint32_t)__popcll(need);
-        if (!exhausted && nneed != 0u && (nneed >= (uint32_t)P.refill_min || nneed == wave_lanes)) {
+        const bool refill = !exhausted && nneed != 0u &&
+                            (nneed >= (uint32_t)P.refill_min || nneed == wave_lanes);
+        if (sess) {
+            // ended photons are taken off their sessions' counts before their lanes take
+            // new photons (or, once no more come, every step); a closed session whose count
+            // reaches zero is written out by this wave
+            if (unreported && (refill || exhausted)) {
+                while (unreported) {
+                    const int q = __builtin_amdgcn_readlane(ev, (int)__builtin_ctzll(unreported));
+                    const uint64_t mq = __ballot(((unreported >> __lane_id()) & 1ull) && ev == q);
+                    const uint32_t n = (uint32_t)__popcll(mq);
+                    uint32_t old = 0u;
+                    if (__lane_id() == 0)
+                        old = __hip_atomic_fetch_sub(&lds[SH_STATE + q], n, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                    old = __builtin_amdgcn_readfirstlane(old);
+                    if (old == SS_CLOSED + n) flush_pending |= 1u << q;
+                    unreported &= ~mq;
+                }
+            }
+            while (flush_pending) {
+                sess_flush(lds, (int)__builtin_ctz(flush_pending), nxy, B);
+                flush_pending &= flush_pending - 1u;
+            }
+        }
+        if (refill) {
             const uint32_t rank = lanes_below(need);
-            const uint64_t avail = pool_end - pool_next;
+            uint64_t p_next = pool_next, p_end = pool_end;
+            int64_t p_dep = pool_dep;
+            int p_slot = pool_slot;
+            const uint64_t avail = p_end - p_next;
             uint64_t base2 = 0, end2 = 0;
             int64_t new_dep = 0;
-            if ((uint64_t)nneed > avail) {
+            int new_slot = 0;
+            if (sess && (uint64_t)nneed > avail) {
+                bool done = false;
+                int flush_q = -1;
+                sess_take(lds, P, B, base2, end2, new_slot, done, flush_q);
+                if (flush_q >= 0) flush_pending |= 1u << flush_q;
+                if (done) exhausted = true;
+                if (end2 > base2) new_dep = find_deposit(B.dep_off, P.ndep, (int64_t)base2);
+            } else if ((uint64_t)nneed > avail) {
                 uint64_t b = 0;
                 if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id()))
                     b = atomicAdd(B.cursor, (unsigned long long)P.chunk);
@@ -146,31 +460,38 @@ transport_kernel(const KParams P, const KBuffers B)
                 else if (MODE == M_EVENTS) new_dep = find_deposit(B.dep_off, P.ndep, (int64_t)b);
             }
             bool take = false;
-            int64_t start_dep = pool_dep;
+            int64_t start_dep = p_dep;
+            int take_slot = p_slot;
             if (!alive) {
-                if ((uint64_t)rank < avail) { k = (int64_t)(pool_next + rank); take = true; }
+                if ((uint64_t)rank < avail) { k = (int64_t)(p_next + rank); take = true; }
                 else {
                     const uint64_t r2 = base2 + ((uint64_t)rank - avail);
-                    if (r2 < end2) { k = (int64_t)r2; take = true; start_dep = new_dep; }
+                    if (r2 < end2) { k = (int64_t)r2; take = true; start_dep = new_dep; take_slot = new_slot; }
                 }
             }
-            if ((uint64_t)nneed <= avail) pool_next += nneed;
+            if ((uint64_t)nneed <= avail) p_next += nneed;
             else {
                 const uint64_t nx2 = base2 + ((uint64_t)nneed - avail);
-                pool_next = nx2 < end2 ? nx2 : end2;
-                pool_end = end2;
-                pool_dep = new_dep;
+                p_next = nx2 < end2 ? nx2 : end2;
+                p_end = end2;
+                p_dep = new_dep;
+                p_slot = new_slot;
             }
+            pool_next = p_next;
+            pool_end = p_end;
+            pool_dep = p_dep;
+            pool_slot = p_slot;
             if (take) {
                 float x0, y0, z0, t0;
                 if (MODE == M_EVENTS) {
                     // photons of a lane only move forward, and the chunk start bounds below
-                    int64_t d = start_dep > my_dep ? start_dep : my_dep;
+                    // (sessions: a chunk lies in one event, so its start is close enough)
+                    int64_t d = sess ? start_dep : (start_dep > my_dep ? start_dep : my_dep);
                     while (B.dep_off[d + 1] <= k) ++d;
-                    my_dep = d;
+                    if (!sess) my_dep = d;
                     const float4 q = B.dep[d];
                     x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
-                    ev = B.dep_event[d];
+                    ev = sess ? take_slot : B.dep_event[d];     // sessions: the map slot
                     t_start = t0;
                     if (MULTIBOX) {
                         // a crystal index outside the table reads crystal 0 (the check
@@ -192,15 +513,18 @@ transport_kernel(const KParams P, const KBuffers B)
                 oc.add(0);
             }
         }
-        if (__ballot(alive) == 0ull) break;         // queue drained, all lanes idle
+        if (!sess && __ballot(alive) == 0ull) break;    // queue drained, all lanes idle
+        const bool idle = sess && __ballot(alive) == 0ull;
 
         // ---- one bounce step ------------------------------------------------------
+        bool died = false;
         if (alive) {
             rng_prefetch(rng, P.key0, P.key1);      // the step's Philox block, one site
             int lin = 0;
             const int st = step<COUNT, OPT>(p, rng, P, oc, lin, nscat, bL0, bL1, bL2);
             if (st != ST_FLYING) {
                 alive = false;
+                died = true;
                 if (MODE == M_RECORDS) {
                     if (B.status) B.status[k] = (uint8_t)st;
                     if (B.idx) B.idx[k] = (int16_t)(st == ST_DETECTED ? lin : 0);
@@ -223,6 +547,13 @@ transport_kernel(const KParams P, const KBuffers B)
                     }
                     const int slot = st == ST_DETECTED ? 0 : (st == ST_CAP ? 5 : (st == ST_ABS_SUR ? 6 : st));
                     atomicAdd(&lds_ctr[slot], 1ull);
+                } else if (sess) {
+                    if (st == ST_DETECTED) {
+                        const int l = lin < 0 ? -lin : lin;
+                        uint32_t *m = lds + SH_MAPS + ev * 2 * nxy;
+                        atomicAdd(&m[l - 1], 1u);
+                        atomicMin(&m[nxy + l - 1], time_key(p.t));
+                    }
                 } else {
                     if (st == ST_DETECTED && (uint32_t)ev < (uint32_t)P.nev) {
                         const int l = lin < 0 ? -lin : lin;
@@ -236,6 +567,25 @@ transport_kernel(const KParams P, const KBuffers B)
                         }
                     }
                 }
+            }
+        }
+        if (sess) {
+            unreported |= __ballot(died);
+            if (idle) {
+                // no more work and all lanes idle: done once every ended photon is reported
+                // and every session this wave closed is written out
+                if (exhausted && !unreported && !flush_pending) break;
+                if (exhausted) continue;
+                // another wave holds the lock, or every slot is draining; a wait that
+                // outlasts the bound (2^27 naps of about 500 cycles, a minute) ends the
+                // wave with the internal-error flag set
+                if (++idle_iters > (1u << 27)) {
+                    if (__lane_id() == 0) atomicOr(B.flag, FLAG_INTERNAL);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            } else {
+                idle_iters = 0;
             }
         }
     }
@@ -270,10 +620,84 @@ __global__ void keys_to_times(const uint32_t *keys, float *t, int64_t n)
         t[i] = key_time(keys[i]);
 }
 
+// A fill that runs only when the device flag's ordering bit says the path it serves is the
+// one taken: want_unsorted 1 for the per-detection path, 0 for owned events.
+__global__ void fill_u32_gated(uint32_t *p, uint32_t v, int64_t n, const unsigned int *flag,
+                               int want_unsorted)
+{
+    if (((*flag & FLAG_UNSORTED) != 0u) != (want_unsorted != 0)) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// first deposit of event >= e in an event-ordered table
+__device__ __forceinline__ int64_t lower_bound_event(const int32_t *ev, int64_t ndep, int64_t e)
+{
+    int64_t lo = 0, hi = ndep;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)ev[mid] < e) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Photon ranges of the events of an event-ordered deposit table, and the partial-map list
+// heads cleared.  Skipped when the device found the table out of order (the other path).
+__global__ void event_ranges(const int32_t *dep_event, const int64_t *off, int64_t ndep,
+                             int64_t nev, unsigned long long *ev_next, unsigned long long *ev_end,
+                             uint32_t *head, const unsigned int *flag, int gated)
+{
+    if (gated && (*flag & FLAG_UNSORTED)) return;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nev;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lo = lower_bound_event(dep_event, ndep, e);
+        const int64_t hi = lower_bound_event(dep_event, ndep, e + 1);
+        ev_next[e] = (unsigned long long)off[lo];
+        ev_end[e] = (unsigned long long)off[hi];
+        head[e] = 0u;
+    }
+}
+
+// Owned events: the helpers' partial maps added to the owners' rows (counts summed, first
+// times the minimum, npe recomputed), one wave per event, and a check that every event
+// was written.
+__global__ void finish_sessions(const uint32_t *head, const uint32_t *link, const uint32_t *pdata,
+                                int32_t *maps, float *tmin, int32_t *npe, int64_t nev, int nxy,
+                                const uint32_t *sess_ctr, unsigned int *flag, int gated)
+{
+    if (gated && (*flag & FLAG_UNSORTED)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (int64_t)sess_ctr[1] != nev)
+        atomicOr(flag, FLAG_INTERNAL);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t e = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; e < nev; e += waves) {
+        const uint32_t h = head[e];
+        if (!h) continue;
+        int32_t s = 0;
+        for (int c = (int)__lane_id(); c < nxy; c += 64) {
+            int32_t cnt = maps[e * nxy + c];
+            uint32_t key = time_key(tmin[e * nxy + c]);
+            for (uint32_t p = h; p; p = link[p - 1]) {
+                const uint32_t *pd = pdata + (int64_t)(p - 1) * 2 * nxy;
+                cnt += (int32_t)pd[c];
+                key = pd[nxy + c] < key ? pd[nxy + c] : key;
+            }
+            maps[e * nxy + c] = cnt;
+            tmin[e * nxy + c] = key_time(key);
+            s += cnt;
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        if (__lane_id() == 0) npe[e] = s;
+    }
+}
+
 // per event: npe = sum of its map (src/events.jl:86), tmin from keys
 __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *tmin,
-                              int32_t *npe, int64_t nev, int nxy)
+                              int32_t *npe, int64_t nev, int nxy, const unsigned int *flag,
+                              int gated)
 {
+    if (gated && !(*flag & FLAG_UNSORTED)) return;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nev;
          e += (int64_t)gridDim.x * blockDim.x) {
         int32_t s = 0;
@@ -319,7 +743,8 @@ __global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, int64_
 // Index checks of a deposit table on the device, for tables that live there: flag bit 2
 // an event index outside [0, nev), bit 4 a crystal index outside [0, nbox) (dep_box may be
 // null).  The transport kernel guards both reads itself; these bits turn a bad table into
-// an error for the caller.
+// an error for the caller.  Bit 8 (not an error) marks a table whose events are not in
+// non-decreasing order: its events are not contiguous photon ranges.
 __global__ void check_deposits(const int32_t *ev, const int32_t *box, int64_t ndep, int64_t nev,
                                int32_t nbox, unsigned int *flag)
 {
@@ -328,6 +753,7 @@ __global__ void check_deposits(const int32_t *ev, const int32_t *box, int64_t nd
          i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t e = ev[i];
         if (e < 0 || (int64_t)e >= nev) bad |= 2u;
+        if (i > 0 && ev[i - 1] > e) bad |= FLAG_UNSORTED;
         if (box) {
             const int32_t b = box[i];
             if (b < 0 || b >= nbox) bad |= 4u;
@@ -419,6 +845,8 @@ struct crysp_ctx {
     int ppt = 4;
     int chunk = 128;
     int refill_min = 12;
+    bool own_events = false;                        // event mode: workgroups own events
+    int piece = 512;                                // photons per workgroup claim of an event
     bool timing = false;
     double kernel_ms = 0.0;
     int64_t n_launches = 0;
@@ -468,7 +896,9 @@ static void *scratch_address(int device)
 
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
-             B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_BOXTAB, B_DEPBOX, B_NBUF };
+             B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_BOXTAB, B_DEPBOX, B_EVNEXT, B_EVEND,
+             B_EVHEAD, B_POOL, B_SESSCTR, B_NBUF };
+static_assert(B_NBUF <= B_NBUF_MAX, "buffer table");
 
 static int ensure(crysp_ctx *c, int id, size_t bytes, void **out)
 {
@@ -577,7 +1007,8 @@ static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
 {
     // persistent grid: ceil(n / (wg*ppt)) workgroups, at most what stays resident
     int64_t want = (n + (int64_t)wg * ppt - 1) / ((int64_t)wg * ppt);
-    int per_cu = std::max(1, c->max_threads_per_cu / wg);
+    // the transport kernels run at most seven waves per SIMD (four SIMDs per CU)
+    int per_cu = std::max(1, std::min(c->max_threads_per_cu, 7 * 4 * 64) / wg);
     if (lds_bytes) per_cu = std::max(1, std::min<int>(per_cu, (int)(160 * 1024 / lds_bytes)));
     int64_t cap = (int64_t)c->n_cu * per_cu;
     return (int)std::max<int64_t>(1, std::min<int64_t>(want, cap));
@@ -587,28 +1018,29 @@ static int grid_for(crysp_ctx *c, int64_t n, int wg, int ppt, size_t lds_bytes)
 // launch's flags (ESR or Teflon over an air gap, Teflon with Rayleigh scattering), else the
 // generic kernel.  Records
 // mode and the op-counting variant always run the generic kernel.
-template <int MODE, bool MULTIBOX = false>
+template <int MODE, bool MULTIBOX = false, bool SESS = false>
 static void launch_optics_variant(const KParams &P, const KBuffers &B, dim3 grid, dim3 block,
                                   size_t lds, hipStream_t stream)
 {
     if constexpr (MODE != M_RECORDS) {
         if (P.flags == OPT_ESR) {
-            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_ESR>), grid, block,
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_ESR, SESS>), grid, block,
                                lds, stream, P, B);
             return;
         }
         if (P.flags == OPT_TEFLON) {
-            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON>), grid, block,
-                               lds, stream, P, B);
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON, SESS>), grid,
+                               block, lds, stream, P, B);
             return;
         }
         if (P.flags == OPT_TEFLON_RAY) {
-            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON_RAY>), grid,
+            hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_TEFLON_RAY, SESS>), grid,
                                block, lds, stream, P, B);
             return;
         }
     }
-    hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX>), grid, block, lds, stream, P, B);
+    hipLaunchKernelGGL((transport_kernel<MODE, false, MULTIBOX, OPT_GENERIC, SESS>), grid, block,
+                       lds, stream, P, B);
 }
 
 template <int MODE>
@@ -736,6 +1168,15 @@ int crysp_set_queue(crysp_ctx *c, int32_t chunk, int32_t refill_min)
     if (refill_min < 1 || refill_min > 64) return fail(CRYSP_EINVAL, "refill_min must be 1..64");
     c->chunk = chunk;
     c->refill_min = refill_min;
+    return CRYSP_OK;
+}
+
+int crysp_set_event_sessions(crysp_ctx *c, int32_t enable, int32_t piece)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    if (piece < 1 || piece > (1 << 24)) return fail(CRYSP_EINVAL, "piece must be 1..2^24");
+    c->own_events = enable != 0;
+    c->piece = piece;
     return CRYSP_OK;
 }
 
@@ -943,10 +1384,13 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                               c->stream));
     }
     const bool ev_dev = ndep > 0 && is_device_ptr(dep_event);
+    bool ev_sorted = true;                          // host tables: events in order
     if (!ev_dev)
-        for (int64_t d = 0; d < ndep; ++d)
+        for (int64_t d = 0; d < ndep; ++d) {
             if (dep_event[d] < 0 || dep_event[d] >= nev)
                 return fail(CRYSP_EINVAL, "dep_event out of range");
+            if (d > 0 && dep_event[d] < dep_event[d - 1]) ev_sorted = false;
+        }
     if ((rc = stage_in(c, B_DEP, dep_xyzt, (size_t)ndep * 16, (const void **)&B.dep)) ||
         (rc = stage_in(c, B_DEPEV, dep_event, (size_t)ndep * 4, (const void **)&B.dep_event)))
         return rc;
@@ -1003,14 +1447,50 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     const size_t lds_ev = ring_bytes(c->wg);
     const int grid = grid_for(c, n_hint, c->wg, c->ppt, lds_ev);
 
+    // Events owned by workgroups (one writer per map row, no global atomics on the maps)
+    // when the table is ordered by event; a device-resident table is checked on the device,
+    // and then both paths are enqueued, each skipping itself unless the flag picks it.
+    const size_t lds_sess = ring_bytes(c->wg) + sess_lds_words(nxy) * 4;
+    const bool sess = c->own_events && !multibox && !tsum_dev && !c->count_ops && !noalloc &&
+                      ndep > 0 && nev > 0 && nxy <= SESS_MAX_CELLS && (ev_dev || ev_sorted) &&
+                      lds_sess + STATIC_LDS_MARGIN <= c->lds_per_block;
+    const bool run_old = !sess || ev_dev;
+    const int grid_sess = sess ? grid_for(c, n_hint, c->wg, c->ppt, lds_sess) : 0;
+    KParams Ps = P;
+    if (sess) {
+        Ps.gate = ev_dev ? 1 : 0;
+        Ps.piece = c->piece;
+        Ps.pool_cap = 8 * grid_sess;
+        Ps.help_window = 2 * (int64_t)grid_sess;
+        P.gate = ev_dev ? 2 : 0;
+        void *p1, *p2, *p3, *p4, *p5;
+        const size_t pool_words = (size_t)Ps.pool_cap * (2 * (size_t)nxy + 1);
+        // the counters, then one open-event word per workgroup
+        if ((rc = ensure(c, B_EVNEXT, (size_t)nev * 8, &p1)) ||
+            (rc = ensure(c, B_EVEND, (size_t)nev * 8, &p2)) ||
+            (rc = ensure(c, B_EVHEAD, (size_t)nev * 4, &p3)) ||
+            (rc = ensure(c, B_POOL, pool_words * 4, &p4)) ||
+            (rc = ensure(c, B_SESSCTR, 16 + (size_t)grid_sess * 4, &p5)))
+            return rc;
+        B.open_ev = (uint32_t *)p5 + 4;
+        B.ev_next = (unsigned long long *)p1;
+        B.ev_end = (const unsigned long long *)p2;
+        B.ev_head = (uint32_t *)p3;
+        B.pool_link = (uint32_t *)p4;
+        B.pool_data = (uint32_t *)p4 + Ps.pool_cap;
+        B.sess_ctr = (uint32_t *)p5;
+        B.tmin_out = (float *)dtmin;
+        B.npe_out = (int32_t *)dnpe;
+    }
+
     // everything the captured nodes read: parameters, every buffer address (the kernel's
     // and the outputs finish_events writes), the launch shape and the variant
     const bool box_dev = multibox && is_device_ptr(dep_box);
-    std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 8 * sizeof(int64_t));
+    std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 10 * sizeof(int64_t));
     {
-        int64_t extra[8] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
-                             (int64_t)c->timing, (int64_t)nph_dev, (int64_t)ev_dev,
-                             (int64_t)box_dev };
+        int64_t extra[10] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
+                              (int64_t)c->timing, (int64_t)nph_dev, (int64_t)ev_dev,
+                              (int64_t)box_dev, (int64_t)sess, (int64_t)c->piece };
         const void *ptrs[3] = { nph_d, dtmin, dnpe };
         unsigned char *q = key.data();
         std::memcpy(q, &P, sizeof P); q += sizeof P;
@@ -1033,10 +1513,26 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
             hipLaunchKernelGGL(check_deposits, dim3((unsigned)std::min<int64_t>(1024, (ndep + 255) / 256)),
                                dim3(256), 0, c->stream, B.dep_event, box_chk, ndep, nev, nbox,
                                (unsigned int *)dflag);
-        hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
-                           (int64_t)cells);
-        hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
-                           KEY_INF, (int64_t)cells);
+        if (run_old && sess) {                      // only if the device picks this path
+            hipLaunchKernelGGL(fill_u32_gated, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps,
+                               0u, (int64_t)cells, (const unsigned int *)dflag, 1);
+            hipLaunchKernelGGL(fill_u32_gated, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
+                               KEY_INF, (int64_t)cells, (const unsigned int *)dflag, 1);
+        } else if (run_old) {
+            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
+                               (int64_t)cells);
+            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
+                               KEY_INF, (int64_t)cells);
+        }
+        if (sess) {
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, B.sess_ctr, 0u, 4);
+            hipLaunchKernelGGL(fill_u32, dim3((grid_sess + 255) / 256), dim3(256), 0, c->stream,
+                               B.open_ev, ~0u, (int64_t)grid_sess);
+            hipLaunchKernelGGL(event_ranges, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
+                               dim3(256), 0, c->stream, B.dep_event, B.dep_off, ndep, nev, B.ev_next,
+                               (unsigned long long *)B.ev_end, B.ev_head, (const unsigned int *)dflag,
+                               (int)ev_dev);
+        }
         if (tsum_dev)
             hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)tsum_dev, 0u,
                                (int64_t)cells * 2);
@@ -1044,7 +1540,11 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
-            if (multibox)
+            if (sess)
+                launch_optics_variant<M_EVENTS, false, true>(Ps, B, dim3(grid_sess), dim3(c->wg),
+                                                             lds_sess, c->stream);
+            if (!run_old) {
+            } else if (multibox)
                 launch_optics_variant<M_EVENTS, true>(P, B, dim3(grid), dim3(c->wg), lds_ev, c->stream);
             else if (c->count_ops)
                 hipLaunchKernelGGL((transport_kernel<M_EVENTS, true>), dim3(grid), dim3(c->wg), lds_ev,
@@ -1053,10 +1553,18 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                 launch_optics_variant<M_EVENTS>(P, B, dim3(grid), dim3(c->wg), lds_ev, c->stream);
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev1, c->stream));
         }
-        if (nev > 0)
-            hipLaunchKernelGGL(finish_events, dim3((unsigned)std::min<int64_t>(1024, (nev + 255) / 256)),
-                               dim3(256), 0, c->stream, (const int32_t *)dmaps, (const uint32_t *)dkeys,
-                               (float *)dtmin, (int32_t *)dnpe, nev, nxy);
+        const dim3 ev_grid((unsigned)std::min<int64_t>(1024, (nev + 255) / 256));
+        if (nev > 0 && run_old)
+            hipLaunchKernelGGL(finish_events, ev_grid, dim3(256), 0, c->stream, (const int32_t *)dmaps,
+                               (const uint32_t *)dkeys, (float *)dtmin, (int32_t *)dnpe, nev, nxy,
+                               (const unsigned int *)dflag, (int)sess);
+        if (sess)
+            hipLaunchKernelGGL(finish_sessions, dim3((unsigned)std::min<int64_t>(4096, (nev + 3) / 4)),
+                               dim3(256), 0, c->stream,
+                               (const uint32_t *)B.ev_head, (const uint32_t *)B.pool_link,
+                               (const uint32_t *)B.pool_data, (int32_t *)dmaps, (float *)dtmin,
+                               (int32_t *)dnpe, nev, nxy, (const uint32_t *)B.sess_ctr,
+                               (unsigned int *)dflag, (int)ev_dev);
         return CRYSP_OK;
     };
     bool launched = false;
@@ -1104,6 +1612,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     if (tail[1] & 1) return fail(CRYSP_EINVAL, "negative photon count in dep_nph");
     if (tail[1] & 2) return fail(CRYSP_EINVAL, "dep_event out of range (those deposits' detections were dropped)");
     if (tail[1] & 4) return fail(CRYSP_EINVAL, "dep_box out of range (those deposits were traced in crystal 0)");
+    if (tail[1] & FLAG_INTERNAL) return fail(CRYSP_EHIP, "internal error: the event queue did not drain");
     if (ndep > 0) return finish_timing(c);
     return CRYSP_OK;
 }

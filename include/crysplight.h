@@ -202,6 +202,16 @@ int crysp_get_schedule(crysp_ctx *ctx, int32_t *wg_size, int32_t *photons_per_th
  * least refill_min of them are idle (or all are).  Defaults 128 and 12. */
 int crysp_set_queue(crysp_ctx *ctx, int32_t chunk, int32_t refill_min);
 
+/* Event-mode work split (no reference counterpart).  enable != 0: when the deposits are
+ * ordered by event (non-decreasing dep_event, as the reference pushes them), workgroups
+ * own whole events, sum their detections in LDS and write each map row once; they claim
+ * an event's photons `piece` at a time (default 512), and workgroups without work help
+ * with events that still have photons.  Tables out of order, several crystals, time sums,
+ * operation counting, grids beyond 256 SiPMs and the allocation-free entry point use
+ * per-detection atomics instead.  enable = 0 (the default) always uses per-detection
+ * atomics.  Outputs are identical either way. */
+int crysp_set_event_sessions(crysp_ctx *ctx, int32_t enable, int32_t piece);
+
 /* Autotuner (no reference counterpart): times crysp_run_events on the given deposits
  * for every (wg_sizes[a], ppts[b]) with `reps` repetitions each, keeps the fastest as
  * the context's schedule and writes the best-of-reps kernel time [ms] of every pair
