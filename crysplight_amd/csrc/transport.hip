@@ -108,6 +108,7 @@ constexpr int SESS_MAX_CELLS = 256;                 // SiPMs a session map holds
 constexpr uint32_t SS_CLOSED = 0x80000000u;         // the event's photons are all claimed
 constexpr uint32_t SS_FREE = 0x40000000u;           // the slot holds no session
 constexpr uint32_t SS_COUNT = 0x3FFFFFFFu;          // photons claimed and not yet ended
+constexpr uint32_t SS_FLUSHING = SS_CLOSED | SS_FREE;  // claimed by the wave writing it out
 constexpr uint32_t CUR_NONE = 0xFFu, CUR_DONE = 0xFEu;
 constexpr uint32_t FLAG_UNSORTED = 8u, FLAG_INTERNAL = 16u;
 // words of the session area in LDS: lock, current slot, the workgroup's claimed photons
@@ -235,15 +236,37 @@ __device__ int64_t find_help(const KParams &P, const KBuffers &B)
     return -1;
 }
 
+// Sessions that are closed with no photon in flight, each claimed (compare-and-swap) by
+// this wave to write out; a bit mask of slots.  Every wave that ends photons looks here
+// after reporting them (or its next take does), so the report that takes a closed session
+// to zero is followed by a scan that finds it.  Wave-uniform.
+__device__ uint32_t sess_scan(uint32_t *S)
+{
+    uint32_t mine = 0u;
+#pragma unroll
+    for (int q = 0; q < NSESS; ++q) {
+        if (lds_uni(&S[SH_STATE + q]) != SS_CLOSED) continue;
+        uint32_t won = 0u;
+        if (__lane_id() == 0) {
+            uint32_t expect = SS_CLOSED;
+            won = __hip_atomic_compare_exchange_strong(&S[SH_STATE + q], &expect, SS_FLUSHING,
+                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP) ? 1u : 0u;
+        }
+        if (__builtin_amdgcn_readfirstlane(won)) mine |= 1u << q;
+    }
+    return mine;
+}
+
 // A wave's next photons from its workgroup's sessions, one action per call: take up to a
 // chunk of the photons the workgroup has claimed ([b, e) of slot q); or claim the next
 // piece of the current session's event (closing the session when the event runs out); or
 // open a session on a new event (an owner) or on one with photons left (a helper); or
 // find there is no more work (done).  b == e when this call took no photons (also when
-// another wave holds the lock).  A session closed with no photon in flight is returned
-// in flush_q for the caller to write out.  Wave-uniform.
+// another wave holds the lock).  Sessions found closed with no photon in flight are
+// claimed into flush for the caller to write out.  Wave-uniform.
 __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint64_t &b,
-                          uint64_t &e, int &q, bool &done, int &flush_q)
+                          uint64_t &e, int &q, bool &done, uint32_t &flush)
 {
     const bool leader = __lane_id() == 0;
     uint32_t locked = 0u;
@@ -275,11 +298,11 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
         const unsigned long long gend = KB(ev_end)[ev];
         const unsigned long long n = g < gend ? (gend - g < piece ? gend - g : piece) : 0ull;
         const bool close = g + piece >= gend;
-        uint32_t old = 0u;
+        // the claimed photons join the session's count (a claim that reached the event's end
+        // closes it; the scan at the end of this call finds it if nothing is in flight)
         if (leader)
-            old = __hip_atomic_fetch_add(&S[SH_STATE + cur], (uint32_t)n + (close ? SS_CLOSED : 0u),
-                                         __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        old = __builtin_amdgcn_readfirstlane(old);
+            __hip_atomic_fetch_add(&S[SH_STATE + cur], (uint32_t)n + (close ? SS_CLOSED : 0u),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (leader) {
             if (n) {
                 const uint64_t n2 = n < (uint64_t)KP(chunk) ? n : (uint64_t)KP(chunk);
@@ -297,7 +320,6 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
             e = g + (n < (uint64_t)KP(chunk) ? n : (uint64_t)KP(chunk));
             q = (int)cur;
         }
-        if (close && (old & SS_COUNT) + (uint32_t)n == 0u) flush_q = (int)cur;
     } else {                                        // open a session in a free slot
         int fq = -1;
         for (int s = 0; s < NSESS; ++s)
@@ -332,6 +354,7 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
             if (ev < 0) done = true;
         }
     }
+    flush |= sess_scan(S);
     if (leader) lds_release(&S[SH_LOCK], 0u);
 }
 
@@ -414,19 +437,18 @@ transport_kernel(const KParams P, const KBuffers B)
             // ended photons are taken off their sessions' counts before their lanes take
             // new photons (or, once no more come, every step); a closed session whose count
             // reaches zero is written out by this wave
+            // (a wave's LDS operations take effect in order, so its detections are in the
+            // session maps before its report; the reports need no reply)
             if (unreported && (refill || exhausted)) {
                 while (unreported) {
                     const int q = __builtin_amdgcn_readlane(ev, (int)__builtin_ctzll(unreported));
                     const uint64_t mq = __ballot(((unreported >> __lane_id()) & 1ull) && ev == q);
-                    const uint32_t n = (uint32_t)__popcll(mq);
-                    uint32_t old = 0u;
                     if (__lane_id() == 0)
-                        old = __hip_atomic_fetch_sub(&lds[SH_STATE + q], n, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-                    old = __builtin_amdgcn_readfirstlane(old);
-                    if (old == SS_CLOSED + n) flush_pending |= 1u << q;
+                        __hip_atomic_fetch_sub(&lds[SH_STATE + q], (uint32_t)__popcll(mq),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     unreported &= ~mq;
                 }
+                if (exhausted) flush_pending |= sess_scan(lds);
             }
             while (flush_pending) {
                 sess_flush(lds, (int)__builtin_ctz(flush_pending), nxy, B);
@@ -444,9 +466,7 @@ transport_kernel(const KParams P, const KBuffers B)
             int new_slot = 0;
             if (sess && (uint64_t)nneed > avail) {
                 bool done = false;
-                int flush_q = -1;
-                sess_take(lds, P, B, base2, end2, new_slot, done, flush_q);
-                if (flush_q >= 0) flush_pending |= 1u << flush_q;
+                sess_take(lds, P, B, base2, end2, new_slot, done, flush_pending);
                 if (done) exhausted = true;
                 if (end2 > base2) new_dep = find_deposit(B.dep_off, P.ndep, (int64_t)base2);
             } else if ((uint64_t)nneed > avail) {
@@ -574,7 +594,13 @@ transport_kernel(const KParams P, const KBuffers B)
             if (idle) {
                 // no more work and all lanes idle: done once every ended photon is reported
                 // and every session this wave closed is written out
-                if (exhausted && !unreported && !flush_pending) break;
+                if (exhausted && !unreported && !flush_pending) {
+                    // a last look: a session this wave's reports took to zero
+                    const uint32_t last = sess_scan(lds);
+                    if (!last) break;
+                    flush_pending |= last;
+                    continue;
+                }
                 if (exhausted) continue;
                 // another wave holds the lock, or every slot is draining; a wait that
                 // outlasts the bound (2^27 naps of about 500 cycles, a minute) ends the
