@@ -63,6 +63,11 @@ def main() -> int:
                     help="fraction of training points drawn within 2 mm of the readout face")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the point-source and batched-uneven-events lines")
+    ap.add_argument("--event-path", choices=("atomic", "owned"), default="atomic",
+                    help="event-mode map updates of the timed steps: a global atomic per "
+                         "detection (default), or workgroup-owned events")
+    ap.add_argument("--piece", type=int, default=1024,
+                    help="workgroup-owned events: photons per claim of an event's photons")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this size once (a plain copy "
                          "kernel of known traffic, to scale rocprofv3 FETCH/WRITE_SIZE)")
@@ -115,6 +120,7 @@ def main() -> int:
         raise RuntimeError(f"op counters inconsistent: {int(counts[0])} photons / {int(counts[8])} "
                            f"detections counted, {n_photons} / {n_det} transported")
 
+    ctx.set_event_sessions(args.event_path == "owned", args.piece)
     ctx.set_timing(True)
     for _ in range(args.warmup):
         step()
@@ -189,6 +195,26 @@ def main() -> int:
         "cpu_baseline": None,
     }
     out["roofline"].update(_traffic())
+    out["tail"] = _tail(ctx, step)
+
+    # the same step with workgroup-owned events (each map row written once, no global
+    # atomic per detection): same maps, its kernel time, roofline and traffic
+    ctx.set_event_sessions(True, args.piece)
+    r = step()
+    if not (np.array_equal(o_maps.cpu().numpy(), maps_gpu) and np.array_equal(o_npe.cpu().numpy(), npe_gpu)):
+        raise RuntimeError("owned-event step differs from the per-detection-atomic step")
+    ctx.set_timing(True)
+    ctx.kernel_time(reset=True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    oms, on = ctx.kernel_time(reset=True)
+    ctx.set_timing(False)
+    ctx.set_event_sessions(False, args.piece)
+    own_s = oms / max(on, 1) / 1e3
+    out["owned_events"] = {"kernel_ms": own_s * 1e3, "frac": ops_per_launch / own_s / opmodel.VALU_PEAK_OPS,
+                           "maps_identical": True}
+    out["owned_events"].update(_traffic("transport_events_bgo_owned_traffic.md"))
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["parity"] = _cpu_baseline(box, op, ro, dep, tau, args,
@@ -201,6 +227,27 @@ def main() -> int:
         out["surrogate"] = _surrogate(args, ctx)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _tail(ctx, launch):
+    """One launch with the wave timeline on: when its waves ended, relative to the launch's
+    start (its HIP-event duration before the last wave's end)."""
+    import numpy as np
+    ctx.set_wave_timeline(True)
+    ctx.set_timing(True)
+    ctx.kernel_time(reset=True)
+    launch()
+    ms, n = ctx.kernel_time(reset=True)
+    before_end = ctx.wave_timeline()
+    ctx.set_wave_timeline(False)
+    ctx.set_timing(False)
+    dur = ms / n / 1e3
+    ends = np.sort(dur - before_end)
+    k90 = int(0.9 * (len(ends) - 1))
+    return {"waves": int(len(ends)), "kernel_ms": dur * 1e3,
+            "waves_running_at_tenths_of_launch": [int((ends > f * dur).sum()) for f in np.arange(10) / 10],
+            "first_wave_done_ms": float(ends[0] * 1e3),
+            "ms_after_90pct_of_waves_done": float((ends[-1] - ends[k90]) * 1e3)}
 
 
 def _other_configs(ctx):
@@ -227,6 +274,7 @@ def _other_configs(ctx):
         return {"ops_per_launch": ops, "achieved_T_lane_ops_per_s": ops / kernel_s / 1e12,
                 "frac": ops / kernel_s / opmodel.VALU_PEAK_OPS}
 
+
     n1 = 1_000_000
     kw = dict(n_photons=n1, seed=1234, pos=(12.8, 12.8, 10.0), tau_ns=1000.0, ctx=ctx)
     ctx.set_op_counting(True)
@@ -245,12 +293,20 @@ def _other_configs(ctx):
     e2e = (time.perf_counter() - t0) / 5
     ms, n = ctx.kernel_time(reset=True)
     ctx.set_timing(False)
+    # why this launch is short of the event-mode rate: ~4 photons per lane, so the launch
+    # ends with the longest photons' step chains, run by a few waves
+    tail1 = _tail(ctx, lambda: T.transport_accumulate(box, op, ro, tb, **kw))
+    rec = T.transport_records(box, op, ro, n_photons=n1, seed=1234, pos=(12.8, 12.8, 10.0),
+                              tau_ns=1000.0, ctx=ctx)
+    b = np.asarray(rec.bounces)
+    tail1["bounces_mean_p99_p9999_max"] = [float(b.mean()), int(np.percentile(b, 99)),
+                                          int(np.percentile(b, 99.99)), int(b.max())]
     res["point_source_csitl"] = {
         "photons": n1, "autotuned_knobs": dict(zip(("wg", "photons_per_thread", "chunk", "refill_min"), best1)),
         "sweep_ms_min_max": [float(times1[np.isfinite(times1)].min()), float(times1[np.isfinite(times1)].max())],
         "kernel_ms": ms / n, "kernel_photons_per_s": n1 / (ms / n / 1e3),
         "end_to_end_photons_per_s": n1 / e2e, "detected_fraction": acc.ndet / n1,
-        "roofline": roofline(ops1, ms / n / 1e3)}
+        "roofline": roofline(ops1, ms / n / 1e3), "tail": tail1}
     ctx.set_schedule(*default_schedule)
     ctx.set_queue(128, 12)
 
@@ -368,10 +424,10 @@ def _surrogate(args, ctx):
             "events": cmp_}
 
 
-def _traffic():
+def _traffic(name="transport_events_bgo_traffic.md"):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
     by a copy-kernel calibration), recorded in profiles/; null when absent."""
-    p = ROOT / "profiles" / "transport_events_bgo_traffic.md"
+    p = ROOT / "profiles" / name
     if not p.exists():
         return {}
     for line in p.read_text().splitlines():

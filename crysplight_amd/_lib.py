@@ -39,6 +39,8 @@ SIGNATURES = {
     "crysp_get_schedule": (C.c_int, [_vp, _vp, _vp]),
     "crysp_set_queue": (C.c_int, [_vp, _i32, _i32]),
     "crysp_set_event_sessions": (C.c_int, [_vp, _i32, _i32]),
+    "crysp_set_wave_timeline": (C.c_int, [_vp, _i32]),
+    "crysp_get_wave_timeline": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "crysp_autotune_events": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64, _u64,
                                         _i32, _pem, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
     "crysp_autotune_events_queue": (C.c_int, [_vp, _pbox, _pop, _pro, _vp, _vp, _vp, _i64, _i64,
@@ -138,7 +140,23 @@ class Context:
     def set_queue(self, chunk: int, refill_min: int):
         check(load().crysp_set_queue(self._h, chunk, refill_min), "crysp_set_queue")
 
-    def set_event_sessions(self, enable: bool, piece: int = 512):
+    def set_wave_timeline(self, enable: bool):
+        check(load().crysp_set_wave_timeline(self._h, int(bool(enable))), "crysp_set_wave_timeline")
+
+    def wave_timeline(self):
+        """Seconds between each wave's end and the end of the last wave of the last launch
+        (waves that never ran dropped), from the device wall clock."""
+        import numpy as np
+        n, hz = C.c_int64(), C.c_double()
+        check(load().crysp_get_wave_timeline(self._h, None, 0, C.byref(n), C.byref(hz)),
+              "crysp_get_wave_timeline")
+        out = np.zeros(max(n.value, 0), np.uint64)
+        check(load().crysp_get_wave_timeline(self._h, out.ctypes.data, n.value, C.byref(n),
+                                             C.byref(hz)), "crysp_get_wave_timeline")
+        out = out[out > 0].astype(np.float64)
+        return (out.max() - out) / hz.value if out.size else out
+
+    def set_event_sessions(self, enable: bool, piece: int = 1024):
         """Event mode: workgroups own whole events (one write per map row) when the
         deposits are ordered by event, or per-detection atomics (enable False)."""
         check(load().crysp_set_event_sessions(self._h, int(bool(enable)), int(piece)),

@@ -55,6 +55,7 @@ struct KBuffers {
     uint32_t *pool_link;                            // per partial: the next one + 1, 0 none
     uint32_t *pool_data;                            // per partial: nxy counts, nxy time keys
     uint32_t *sess_ctr;                             // [0] partials used, [1] events written
+    const uint32_t *ev_order;                       // events in the order owners claim them
     uint32_t *open_ev;                              // per workgroup: the event it owns and has
                                                     // not yet claimed to the end, or ~0
     float *tmin_out;                                // tmin (f32), written by the event's owner
@@ -201,7 +202,8 @@ __device__ void sess_flush(uint32_t *S, int q, int nxy, const KBuffers &B)
 }
 
 // An event a helper can join: one that its owner has not yet claimed to the end (the
-// owners publish their current event) with at least a chunk of photons unclaimed.  The
+// owners publish their current event) with at least two pieces of photons unclaimed, so
+// that a helper's partial map pays for itself.  The
 // scan starts past this workgroup's own entry, four entries per lane at a time; -1 when no
 // owner has such an event.  Wave-uniform.
 __device__ int64_t find_help(const KParams &P, const KBuffers &B)
@@ -228,7 +230,7 @@ __device__ int64_t find_help(const KParams &P, const KBuffers &B)
                 const int64_t e = (int64_t)__builtin_amdgcn_readlane(evs[i], l);
                 const unsigned long long nx =
                     __hip_atomic_load(&KB(ev_next)[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (nx + (unsigned long long)KP(chunk) <= KB(ev_end)[e]) return e;
+                if (nx + 2ull * (unsigned long long)KP(piece) <= KB(ev_end)[e]) return e;
                 mm &= mm - 1;
             }
         }
@@ -290,8 +292,9 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
     } else if (cur < (uint32_t)NSESS) {             // the next piece of the session's event
         const int64_t ev = (int32_t)lds_uni(&S[SH_EV + cur]);
         const bool helper = (int32_t)lds_uni(&S[SH_PART + cur]) >= 0;
-        // owners claim pieces; helpers, who share the event with its owner, a chunk at a time
-        const unsigned long long piece = (unsigned long long)(helper ? KP(chunk) : KP(piece));
+        // owners claim pieces; helpers, who share the event with its owner, half pieces
+        const unsigned long long piece =
+            (unsigned long long)(helper ? (KP(piece) / 2 > KP(chunk) ? KP(piece) / 2 : KP(chunk)) : KP(piece));
         unsigned long long g = 0;
         if (leader) g = atomicAdd(&KB(ev_next)[ev], piece);
         g = first_lane_u64(g);
@@ -328,7 +331,9 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
             unsigned long long ec = 0;
             if (leader) ec = atomicAdd(KB(cursor), 1ull);
             ec = first_lane_u64(ec);
-            int64_t ev = (int64_t)ec;               // the event's owner
+            int64_t ev = -1;
+            if (ec < (unsigned long long)KP(nev))   // the event's owner
+                ev = (int64_t)__builtin_amdgcn_readfirstlane(KB(ev_order)[ec]);
             int32_t part = -1;
             if (ec >= (unsigned long long)KP(nev)) {
                 ev = find_help(P, B);
@@ -358,6 +363,12 @@ __device__ void sess_take(uint32_t *S, const KParams &P, const KBuffers &B, uint
     if (leader) lds_release(&S[SH_LOCK], 0u);
 }
 
+// Wave timeline (diagnostics): per wave of the last launch, the tick of the device wall
+// clock at which it ended, when g_timeline_on is set.  Every context on a device shares them.
+constexpr uint32_t TL_MAX_WAVES = 16384;
+__device__ unsigned long long g_timeline[TL_MAX_WAVES];
+__device__ int g_timeline_on;
+
 // Seven waves per SIMD (at most 72 VGPRs): the waves cover the LDS reads of the Philox
 // rings and each other's dependent binary64 chains.
 template <int MODE, bool COUNT, bool MULTIBOX = false, uint32_t OPT = OPT_GENERIC,
@@ -378,6 +389,7 @@ transport_kernel(const KParams P, const KBuffers B)
             (__hip_atomic_load(B.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_UNSORTED) != 0u;
         if (unsorted != (P.gate == 2)) return;
     }
+
     if (sess) {
         for (int w = threadIdx.x; w < (int)sess_lds_words(nxy); w += blockDim.x) {
             uint32_t v = 0u;
@@ -616,6 +628,12 @@ transport_kernel(const KParams P, const KBuffers B)
         }
     }
 
+    // wave timeline (diagnostics): this wave's end on the device wall clock (a file-scope
+    // array: its address is a constant, so nothing stays live across the photon loop)
+    if (g_timeline_on && __lane_id() == 0) {
+        const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (w < TL_MAX_WAVES) g_timeline[w] = wall_clock64();
+    }
     if (COUNT) {
         __syncthreads();
         if (threadIdx.x < 10 && lds_ops[threadIdx.x]) atomicAdd(&B.opcounts[threadIdx.x], lds_ops[threadIdx.x]);
@@ -683,6 +701,34 @@ __global__ void event_ranges(const int32_t *dep_event, const int64_t *off, int64
         ev_end[e] = (unsigned long long)off[hi];
         head[e] = 0u;
     }
+}
+
+// The order in which owners claim events: largest first (by photon count, in powers of
+// two), so that the events still running when the last one has an owner are small ones
+// and the launch ends with little left to split among helpers.  One workgroup; a counting
+// sort over 65 size classes.  Skipped with the other path.
+__global__ void __launch_bounds__(1024) order_events(const unsigned long long *ev_next,
+                                                     const unsigned long long *ev_end, int64_t nev,
+                                                     uint32_t *order, const unsigned int *flag,
+                                                     int gated)
+{
+    if (gated && (*flag & FLAG_UNSORTED)) return;
+    __shared__ uint32_t cnt[65];
+    auto size_class = [&](int64_t e) {
+        const unsigned long long n = ev_end[e] - ev_next[e];
+        return n ? 64 - __clzll((long long)n) : 0;
+    };
+    for (int i = threadIdx.x; i < 65; i += blockDim.x) cnt[i] = 0u;
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < nev; e += blockDim.x) atomicAdd(&cnt[size_class(e)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0u;
+        for (int b = 64; b >= 0; --b) { const uint32_t c = cnt[b]; cnt[b] = acc; acc += c; }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < nev; e += blockDim.x)
+        order[atomicAdd(&cnt[size_class(e)], 1u)] = (uint32_t)e;
 }
 
 // Owned events: the helpers' partial maps added to the owners' rows (counts summed, first
@@ -872,7 +918,11 @@ struct crysp_ctx {
     int chunk = 128;
     int refill_min = 12;
     bool own_events = false;                        // event mode: workgroups own events
-    int piece = 512;                                // photons per workgroup claim of an event
+    bool timeline = false;                          // record each wave's start and end
+    int64_t timeline_waves = 0;                     // waves of the last recorded launch
+    void *timeline_dev = nullptr;                   // the device's g_timeline
+    int64_t wall_hz = 100000000;                    // device wall clock rate
+    int piece = 1024;                               // photons per workgroup claim of an event
     bool timing = false;
     double kernel_ms = 0.0;
     int64_t n_launches = 0;
@@ -923,7 +973,7 @@ static void *scratch_address(int device)
 enum BufId { B_PX, B_PY, B_PZ, B_PT, B_STATUS, B_IDX, B_TDET, B_BNC, B_COUNTS, B_TKEYS,
              B_COUNTERS, B_CURSOR, B_DEP, B_NPH, B_DEPEV, B_DEPOFF, B_MAPS, B_TMIN, B_EVKEYS,
              B_NPE, B_OPS, B_MISC, B_MISC2, B_FLAG, B_BOXTAB, B_DEPBOX, B_EVNEXT, B_EVEND,
-             B_EVHEAD, B_POOL, B_SESSCTR, B_NBUF };
+             B_EVHEAD, B_POOL, B_SESSCTR, B_EVORDER, B_NBUF };
 static_assert(B_NBUF <= B_NBUF_MAX, "buffer table");
 
 static int ensure(crysp_ctx *c, int id, size_t bytes, void **out)
@@ -1069,6 +1119,17 @@ static void launch_optics_variant(const KParams &P, const KBuffers &B, dim3 grid
                        lds, stream, P, B);
 }
 
+// The wave timeline buffer for a launch of `waves` waves (cleared by a fill kernel), or
+// nothing when the timeline is off.
+static int timeline_setup(crysp_ctx *c, int64_t waves)
+{
+    if (!c->timeline) return CRYSP_OK;
+    c->timeline_waves = std::min<int64_t>(waves, TL_MAX_WAVES);
+    hipLaunchKernelGGL(fill_u32, dim3(64), dim3(256), 0, c->stream, (uint32_t *)c->timeline_dev, 0u,
+                       (int64_t)TL_MAX_WAVES * 2);
+    return CRYSP_OK;
+}
+
 template <int MODE>
 static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_bytes)
 {
@@ -1082,6 +1143,7 @@ static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_byte
     if (rc) return rc;
     B.cursor = (unsigned long long *)cur;
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
+    if ((rc = timeline_setup(c, (int64_t)grid * (wg / 64)))) return rc;
     if (c->count_ops) {
         void *ops;
         rc = ensure(c, B_OPS, 10 * sizeof(unsigned long long), &ops);
@@ -1149,6 +1211,10 @@ crysp_ctx *crysp_create(int device)
     c->n_cu = prop.multiProcessorCount;
     c->max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
     c->lds_per_block = prop.sharedMemPerBlock;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+        wall_khz > 0)
+        c->wall_hz = (int64_t)wall_khz * 1000;
     c->scratch = scratch_address(device);           // device current (hipSetDevice above)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -1194,6 +1260,35 @@ int crysp_set_queue(crysp_ctx *c, int32_t chunk, int32_t refill_min)
     if (refill_min < 1 || refill_min > 64) return fail(CRYSP_EINVAL, "refill_min must be 1..64");
     c->chunk = chunk;
     c->refill_min = refill_min;
+    return CRYSP_OK;
+}
+
+int crysp_set_wave_timeline(crysp_ctx *c, int32_t enable)
+{
+    if (!c) return fail(CRYSP_EINVAL, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->timeline_dev)
+        HIPCHK(hipGetSymbolAddress(&c->timeline_dev, HIP_SYMBOL(g_timeline)));
+    const int on = enable != 0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_timeline_on), &on, sizeof on));
+    c->timeline = on != 0;
+    return CRYSP_OK;
+}
+
+int crysp_get_wave_timeline(crysp_ctx *c, uint64_t *out, int64_t cap, int64_t *n_waves,
+                            double *ticks_per_s)
+{
+    if (!c || (cap > 0 && !out)) return fail(CRYSP_EINVAL, "bad timeline arguments");
+    const int64_t n = c->timeline_waves;
+    if (n_waves) *n_waves = n;
+    if (ticks_per_s) *ticks_per_s = (double)c->wall_hz;
+    const int64_t m = std::min<int64_t>(n, std::max<int64_t>(cap, 0));
+    if (m > 0 && c->timeline_dev) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(out, c->timeline_dev, (size_t)m * 8, hipMemcpyDeviceToHost));
+    }
     return CRYSP_OK;
 }
 
@@ -1498,6 +1593,9 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
             (rc = ensure(c, B_POOL, pool_words * 4, &p4)) ||
             (rc = ensure(c, B_SESSCTR, 16 + (size_t)grid_sess * 4, &p5)))
             return rc;
+        void *p6;
+        if ((rc = ensure(c, B_EVORDER, (size_t)nev * 4, &p6))) return rc;
+        B.ev_order = (const uint32_t *)p6;
         B.open_ev = (uint32_t *)p5 + 4;
         B.ev_next = (unsigned long long *)p1;
         B.ev_end = (const unsigned long long *)p2;
@@ -1512,6 +1610,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     // everything the captured nodes read: parameters, every buffer address (the kernel's
     // and the outputs finish_events writes), the launch shape and the variant
     const bool box_dev = multibox && is_device_ptr(dep_box);
+    const int64_t tl_waves = (int64_t)std::max(grid, grid_sess) * (c->wg / 64);
     std::vector<unsigned char> key(sizeof P + sizeof B + 3 * sizeof(void *) + 10 * sizeof(int64_t));
     {
         int64_t extra[10] = { (int64_t)grid, (int64_t)c->wg, nev, (int64_t)c->count_ops,
@@ -1558,12 +1657,16 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                                dim3(256), 0, c->stream, B.dep_event, B.dep_off, ndep, nev, B.ev_next,
                                (unsigned long long *)B.ev_end, B.ev_head, (const unsigned int *)dflag,
                                (int)ev_dev);
+            hipLaunchKernelGGL(order_events, dim3(1), dim3(1024), 0, c->stream,
+                               (const unsigned long long *)B.ev_next, B.ev_end, nev,
+                               (uint32_t *)B.ev_order, (const unsigned int *)dflag, (int)ev_dev);
         }
         if (tsum_dev)
             hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)tsum_dev, 0u,
                                (int64_t)cells * 2);
         if (c->count_ops)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
+        if ((rc = timeline_setup(c, tl_waves))) return rc;
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));
             if (sess)

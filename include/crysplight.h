@@ -205,12 +205,23 @@ int crysp_set_queue(crysp_ctx *ctx, int32_t chunk, int32_t refill_min);
 /* Event-mode work split (no reference counterpart).  enable != 0: when the deposits are
  * ordered by event (non-decreasing dep_event, as the reference pushes them), workgroups
  * own whole events, sum their detections in LDS and write each map row once; they claim
- * an event's photons `piece` at a time (default 512), and workgroups without work help
- * with events that still have photons.  Tables out of order, several crystals, time sums,
+ * an event's photons `piece` at a time (default 1024), take events largest first, and
+ * workgroups without work help with events that still have photons.  Tables out of order, several crystals, time sums,
  * operation counting, grids beyond 256 SiPMs and the allocation-free entry point use
  * per-detection atomics instead.  enable = 0 (the default) always uses per-detection
  * atomics.  Outputs are identical either way. */
 int crysp_set_event_sessions(crysp_ctx *ctx, int32_t enable, int32_t piece);
+
+/* Wave timeline (diagnostics, no reference counterpart): with enable != 0 every transport
+ * launch records, per wave, the device wall-clock tick at which the wave ended.
+ * crysp_get_wave_timeline copies the last launch's ticks of up to `cap` waves into out
+ * (cap u64; 0 for a wave that never ran), and sets *n_waves (waves of that launch) and
+ * *ticks_per_s (the clock's rate).  With the launch's duration it shows how long the
+ * launch ran with few waves left: the tail that the longest photons set.  All contexts on
+ * a device share one record. */
+int crysp_set_wave_timeline(crysp_ctx *ctx, int32_t enable);
+int crysp_get_wave_timeline(crysp_ctx *ctx, uint64_t *out, int64_t cap, int64_t *n_waves,
+                            double *ticks_per_s);
 
 /* Autotuner (no reference counterpart): times crysp_run_events on the given deposits
  * for every (wg_sizes[a], ppts[b]) with `reps` repetitions each, keeps the fastest as

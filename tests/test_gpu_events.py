@@ -394,7 +394,7 @@ def _with_sessions(ctx, enable, piece, fn):
     try:
         return fn()
     finally:
-        ctx.set_event_sessions(False, 512)
+        ctx.set_event_sessions(False, 1024)
 
 
 @pytest.mark.parametrize("piece", [1, 37, 512, 1 << 20])
@@ -483,3 +483,21 @@ def test_unordered_tables_take_the_atomic_path(ctx, bgo_events):
             K.C2_BOX, K.C2_OP, RO8, shuf, seed=7, tau_ns=K.C2_TAU, ctx=ctx, device_arrays=darr))
         assert np.array_equal(np.asarray(res.maps).reshape(dep.n_events, -1), ref["maps"]), rep
         assert np.array_equal(np.asarray(res.npe), ref["npe"]), rep
+
+
+def test_wave_timeline(ctx, bgo_events):
+    """The diagnostics timeline: every wave of the last launch reports its end; the spread
+    of the ends lies within the launch."""
+    dep, _ = bgo_events
+    ctx.set_wave_timeline(True)
+    ctx.set_timing(True)
+    ctx.kernel_time(reset=True)
+    try:
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx)
+        ms, n = ctx.kernel_time(reset=True)
+        before_end = ctx.wave_timeline()
+    finally:
+        ctx.set_wave_timeline(False)
+        ctx.set_timing(False)
+    assert before_end.size > 0 and before_end.min() == 0.0
+    assert before_end.max() <= ms / n / 1e3
