@@ -577,6 +577,68 @@ __device__ __forceinline__ void emit(Rng &r, const KParams &P, OpCount<COUNT> &o
     if (P.tau_r > 0.0f) t += -P.tau_r * f_log(randu<COUNT>(r, P, oc));
 }
 
+// Emission in batches.  A wave computes the emissions of the next (up to) 64 photons it
+// will hand out with all its lanes at once, instead of a few lanes at each refill: lane i
+// generates block 0 of its photon's stream and makes emit()'s draws from it.  The results
+// wait in the wave's slots in LDS (word-major: word j of slot s at buf[64 j + s]; the
+// caller keeps a sixth word per slot): the direction, then two words A, B that hold the
+// emission delays or the block's words the emission did not draw (when a delay is off,
+// its draw is not made):
+//   decay and rise: A = decay delay, B = rise delay   (4 draws)
+//   decay only:     A = decay delay, B = word 3       (3 draws)
+//   rise only:      A = rise delay,  B = word 3       (3 draws)
+//   neither:        A = word 2,      B = word 3       (2 draws)
+// The lane that takes the photon restores its stream where emit() would have left it:
+// block 1 next, the undrawn words in their ring slots.  Directions and times are the same
+// operations on the same words, so the photon is the one emit() makes.
+constexpr uint32_t EMB_SLOTS = 64;
+constexpr uint32_t EMB_HDR = 6 * EMB_SLOTS;               // header words after the slots
+constexpr uint32_t EMB_WAVE_BYTES = (EMB_HDR + 16) * 4;   // a wave's slots and header
+
+__device__ __forceinline__ void emit_batch_slot(const KParams &P, uint64_t pid, uint32_t *buf,
+                                                uint32_t s)
+{
+    const uint4 w = philox10((uint32_t)pid, (uint32_t)(pid >> 32), 0u, 0u, P.key0, P.key1);
+    const float z = 1.0f - 2.0f * unit_of(w.x);
+    const float rr = f_sqrt(jmax0(1.0f - z * z));
+    const float phi = TWO_PI_F * unit_of(w.y);
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    uint32_t A = w.z, Bw = w.w;
+    if (P.tau_d > 0.0f) {
+        A = __float_as_uint(-P.tau_d * f_log(unit_of(w.z)));
+        if (P.tau_r > 0.0f) Bw = __float_as_uint(-P.tau_r * f_log(unit_of(w.w)));
+    } else if (P.tau_r > 0.0f) {
+        A = __float_as_uint(-P.tau_r * f_log(unit_of(w.z)));
+    }
+    buf[s] = __float_as_uint(rr * cp);
+    buf[EMB_SLOTS + s] = __float_as_uint(rr * sp);
+    buf[2 * EMB_SLOTS + s] = __float_as_uint(z);
+    buf[3 * EMB_SLOTS + s] = A;
+    buf[4 * EMB_SLOTS + s] = Bw;
+}
+
+__device__ __forceinline__ void emit_take(Rng &r, const KParams &P, uint64_t pid,
+                                          const uint32_t *buf, uint32_t s, float t0,
+                                          float &ux, float &uy, float &uz, float &t)
+{
+    ux = __uint_as_float(buf[s]);
+    uy = __uint_as_float(buf[EMB_SLOTS + s]);
+    uz = __uint_as_float(buf[2 * EMB_SLOTS + s]);
+    const uint32_t A = buf[3 * EMB_SLOTS + s], Bw = buf[4 * EMB_SLOTS + s];
+    const bool dec = P.tau_d > 0.0f, ris = P.tau_r > 0.0f;
+    const uint32_t m = 2u + (dec ? 1u : 0u) + (ris ? 1u : 0u);
+    rng_init(r, pid);
+    r.blk = 1u;
+    r.rd = m * RING_SLOT;
+    r.wr = 4u * RING_SLOT;
+    if (m <= 3u) *(uint32_t *)(r.ring + ring_addr(r, 3u * RING_SLOT)) = Bw;
+    if (m == 2u) *(uint32_t *)(r.ring + ring_addr(r, 2u * RING_SLOT)) = A;
+    t = t0;
+    if (dec) t += __uint_as_float(A);
+    if (ris) t += __uint_as_float(dec ? Bw : A);
+}
+
 struct Photon {
     float x, y, z, ux, uy, uz, t;
     int32_t bounces;

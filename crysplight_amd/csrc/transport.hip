@@ -62,6 +62,14 @@ struct KBuffers {
     int32_t *npe_out;
 };
 
+// v, which the compiler cannot see through (a value derived from it is recomputed where
+// it is used instead of being kept in a register)
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
@@ -379,10 +387,14 @@ transport_kernel(const KParams P, const KBuffers B)
     // dynamic LDS: the lanes' Philox rings (blockDim.x * 64 B), then in accumulate mode
     // the histogram and time keys
     extern __shared__ uint32_t lds_dyn[];
-    uint32_t *lds = lds_dyn + 16u * blockDim.x;
+    constexpr bool sess = MODE == M_EVENTS && SESS;
+    // emission in batches (not in the counting variant or with sessions); the kernels
+    // without sessions keep each wave's batch after the rings
+    constexpr bool emb = !COUNT && !sess;
+    uint32_t *lds = (uint32_t *)((char *)lds_dyn + 64u * blockDim.x +
+                                 (sess ? 0u : (blockDim.x >> 6) * EMB_WAVE_BYTES));
     __shared__ unsigned long long lds_ctr[9];       // accumulate: the Accumulator counters
     const int nxy = P.nx * P.ny;
-    constexpr bool sess = MODE == M_EVENTS && SESS;
     if (MODE == M_EVENTS && P.gate) {
         // the device decides between the two event paths: ordered deposits or not
         const bool unsorted =
@@ -438,6 +450,54 @@ transport_kernel(const KParams P, const KBuffers B)
     const uint64_t N = (MODE == M_EVENTS && P.n_photons < 0) ? (uint64_t)B.dep_off[P.ndep]
                                                              : (uint64_t)P.n_photons;
     const uint32_t wave_lanes = (uint32_t)__popcll(__ballot(true));
+    // The wave's batch: EMB_SLOTS slots of six words (word-major: the emission's five
+    // words, then the photon's deposit relative to the batch's first), then a header
+    // (in 64-bit words): the photons of the last chunk not yet emitted start at
+    // res_next (in deposit res_dep; res_left of them); the emitted batch starts at
+    // photon eb_lo (in deposit eb_dep), holds cnt photons, of which pos are handed out.
+    // Addressed at use from the ring's lane offset (wave w's ring starts at byte
+    // 4096 w), through an empty asm so that no register holds the address across the
+    // loop; the header lives in LDS for the same reason.
+#define EMB_BUF ((uint32_t *)((char *)lds_dyn + 64u * blockDim.x + (opaque_u32(rng.base) >> 12) * EMB_WAVE_BYTES))
+    enum { EH_RES_NEXT = 0, EH_RES_DEP = 2, EH_EB_LO = 4, EH_EB_DEP = 6, EH_RES_LEFT = 8,
+           EH_POS = 9, EH_CNT = 10 };
+    if (emb) {
+        uint32_t *h = EMB_BUF + EMB_HDR;
+        // every lane writes the same words: a lane reads back what it wrote itself
+        lds_set_u64(h + EH_RES_NEXT, 0ull);
+        lds_set_u64(h + EH_RES_DEP, 0ull);
+        lds_set_u64(h + EH_EB_LO, 0ull);
+        lds_set_u64(h + EH_EB_DEP, 0ull);
+        h[EH_RES_LEFT] = 0u;
+        h[EH_POS] = 0u;
+        h[EH_CNT] = 0u;
+    }
+    // a lane takes batch slot `slot`
+    auto emb_take = [&](uint32_t slot) {
+        const uint32_t *b = EMB_BUF;
+        k = (int64_t)(lds_u64(b + EMB_HDR + EH_EB_LO) + slot);
+        float x0, y0, z0, t0;
+        if (MODE == M_EVENTS) {
+            const int64_t d = (int64_t)lds_u64(b + EMB_HDR + EH_EB_DEP) + b[5 * EMB_SLOTS + slot];
+            const float4 q = B.dep[d];
+            x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
+            ev = B.dep_event[d];
+            t_start = t0;
+            if (MULTIBOX) {
+                const int32_t bi = B.dep_box[d];
+                const float4 bx = B.boxes[(uint32_t)bi < (uint32_t)P.nbox ? bi : 0];
+                bL0 = bx.x; bL1 = bx.y; bL2 = bx.z;
+            }
+        } else {
+            const int64_t j = P.npos == 1 ? 0 : k % (int64_t)P.npos;
+            x0 = B.px[j]; y0 = B.py[j]; z0 = B.pz[j];
+            t0 = B.pt ? B.pt[j] : 0.0f;
+        }
+        p.x = x0; p.y = y0; p.z = z0; p.bounces = 0;
+        emit_take(rng, P, P.pid0 + (uint64_t)k + 1ull, b, slot, t0, p.ux, p.uy, p.uz, p.t);
+        nscat = 0;
+        alive = true;
+    };
 
     for (;;) {
         // ---- refill idle lanes, in batches of at least refill_min ------------------
@@ -467,7 +527,68 @@ transport_kernel(const KParams P, const KBuffers B)
                 flush_pending &= flush_pending - 1u;
             }
         }
-        if (refill) {
+        if (emb && refill) {
+            // photons come from the wave's emitted batch; lanes past its end take from
+            // the next batch, which the whole wave emits
+            const uint32_t rank = lanes_below(need);
+            uint32_t pos = lds_uni(EMB_BUF + EMB_HDR + EH_POS);
+            const uint32_t avail = lds_uni(EMB_BUF + EMB_HDR + EH_CNT) - pos;
+            if (!alive && rank < avail) emb_take(pos + rank);
+            if (nneed > avail) {
+                uint32_t *h = EMB_BUF + EMB_HDR;
+                uint64_t rn = lds_u64(h + EH_RES_NEXT);
+                int64_t rd = (int64_t)lds_u64(h + EH_RES_DEP);
+                uint32_t left = lds_uni(h + EH_RES_LEFT);
+                if (left == 0u) {                           // the next chunk of the queue
+                    uint64_t b = 0;
+                    if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id()))
+                        b = atomicAdd(B.cursor, (unsigned long long)P.chunk);
+                    b = first_lane_u64(b);
+                    if (b >= N) exhausted = true;
+                    else {
+                        rn = b;
+                        left = N - b < (uint64_t)P.chunk ? (uint32_t)(N - b) : (uint32_t)P.chunk;
+                        if (MODE == M_EVENTS) rd = find_deposit(B.dep_off, P.ndep, (int64_t)b);
+                    }
+                }
+                const uint32_t nb = left < EMB_SLOTS ? left : EMB_SLOTS;
+                // the takes above read the old batch before the wave writes the new one,
+                // and the takes below read slots other lanes wrote (a wave's LDS
+                // operations complete in order; the fences keep the compiler's order)
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                uint32_t last = 0u;
+                if (__lane_id() < nb) {
+                    const uint64_t kk = rn + __lane_id();
+                    emit_batch_slot(P, P.pid0 + kk + 1ull, EMB_BUF, __lane_id());
+                    if (MODE == M_EVENTS) {
+                        // the deposit of photon kk: at or after the chunk's, a few at most
+                        int64_t d = rd;
+                        while (B.dep_off[d + 1] <= (int64_t)kk) ++d;
+                        last = (uint32_t)(d - rd);
+                        EMB_BUF[5 * EMB_SLOTS + __lane_id()] = last;
+                    }
+                }
+                h = EMB_BUF + EMB_HDR;
+                lds_set_u64(h + EH_EB_LO, rn);
+                lds_set_u64(h + EH_EB_DEP, (uint64_t)rd);
+                if (nb) {
+                    if (MODE == M_EVENTS) rd += __builtin_amdgcn_readlane(last, (int)nb - 1);
+                    rn += nb;
+                }
+                lds_set_u64(h + EH_RES_NEXT, rn);
+                lds_set_u64(h + EH_RES_DEP, (uint64_t)rd);
+                h[EH_RES_LEFT] = left - nb;
+                h[EH_CNT] = nb;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const uint32_t r2 = rank - avail;
+                if (!alive && rank >= avail && r2 < nb) emb_take(r2);
+                const uint32_t want = nneed - avail;
+                pos = want < nb ? want : nb;
+            } else {
+                pos += nneed;
+            }
+            EMB_BUF[EMB_HDR + EH_POS] = pos;
+        } else if (refill) {
             const uint32_t rank = lanes_below(need);
             uint64_t p_next = pool_next, p_end = pool_end;
             int64_t p_dep = pool_dep;
@@ -631,7 +752,7 @@ transport_kernel(const KParams P, const KBuffers B)
     // wave timeline (diagnostics): this wave's end on the device wall clock (a file-scope
     // array: its address is a constant, so nothing stays live across the photon loop)
     if (g_timeline_on && __lane_id() == 0) {
-        const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (opaque_u32(rng.base) >> 12);
         if (w < TL_MAX_WAVES) g_timeline[w] = wall_clock64();
     }
     if (COUNT) {
@@ -648,6 +769,7 @@ transport_kernel(const KParams P, const KBuffers B)
             if (lds[P.hist_words + w] != KEY_INF) atomicMin(&B.tkeys[w], lds[P.hist_words + w]);
     }
 }
+#undef EMB_BUF
 
 // -------------------------------------------------------------- small kernels ----
 __global__ void fill_u32(uint32_t *p, uint32_t v, int64_t n)
@@ -1076,6 +1198,8 @@ static int fill_params(KParams &P, const crysp_box *box, const crysp_optical_par
 
 // LDS of the lanes' Philox rings: 16 words per lane
 static size_t ring_bytes(int wg) { return (size_t)wg * 64; }
+// the batched-emission slots and header of the kernels without sessions, per wave
+static size_t emb_bytes(int wg) { return (size_t)(wg / 64) * EMB_WAVE_BYTES; }
 // the transport kernel's static LDS (counters) stays below this
 constexpr size_t STATIC_LDS_MARGIN = 256;
 
@@ -1134,7 +1258,7 @@ template <int MODE>
 static int launch_transport(crysp_ctx *c, KParams P, KBuffers B, size_t lds_bytes)
 {
     const int wg = c->wg;
-    lds_bytes += ring_bytes(wg);
+    lds_bytes += ring_bytes(wg) + emb_bytes(wg);
     const int grid = grid_for(c, P.n_photons, wg, c->ppt, lds_bytes);
     P.chunk = c->chunk;
     P.refill_min = c->refill_min;
@@ -1243,9 +1367,10 @@ int crysp_set_schedule(crysp_ctx *c, int32_t wg, int32_t ppt)
 {
     if (!c) return fail(CRYSP_EINVAL, "null context");
     if (wg < 64 || wg > 1024 || wg % 64) return fail(CRYSP_EINVAL, "wg_size must be 64..1024, multiple of 64");
-    if (ring_bytes(wg) + STATIC_LDS_MARGIN > c->lds_per_block)
+    if (ring_bytes(wg) + emb_bytes(wg) + STATIC_LDS_MARGIN > c->lds_per_block)
         return fail(CRYSP_ERANGE, "wg_size " + std::to_string(wg) + " needs " +
-                                  std::to_string(ring_bytes(wg)) + " B of LDS for the Philox rings; "
+                                  std::to_string(ring_bytes(wg) + emb_bytes(wg)) +
+                                  " B of LDS for the Philox rings and emission slots; "
                                   "the device allows " + std::to_string(c->lds_per_block) + " B per workgroup");
     if (ppt < 1 || ppt > 64) return fail(CRYSP_EINVAL, "photons_per_thread must be 1..64");
     c->wg = wg;
@@ -1399,8 +1524,8 @@ int crysp_transport_accumulate(crysp_ctx *c, const crysp_box *box,
     const int64_t nxy = (int64_t)ro->grid.nx * ro->grid.ny;
     const int64_t hist = nxy * ((int64_t)tb->nbins + 1);
     const size_t lds_bytes = (size_t)(hist + nxy) * 4;
-    if (lds_bytes + ring_bytes(c->wg) + STATIC_LDS_MARGIN > c->lds_per_block)
-        return fail(CRYSP_ERANGE, "nx*ny*(nbins+2) histogram words plus 16 words per lane exceed "
+    if (lds_bytes + ring_bytes(c->wg) + emb_bytes(c->wg) + STATIC_LDS_MARGIN > c->lds_per_block)
+        return fail(CRYSP_ERANGE, "nx*ny*(nbins+2) histogram words plus the rings and emission batches (89 B per lane) exceed "
                                   "the workgroup LDS (" + std::to_string(c->lds_per_block) + " B)");
     P.n_photons = n;
     P.npos = npos;
@@ -1565,7 +1690,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     // the grid only sizes the persistent launch; the kernel drains whatever total it reads
     const int64_t n_hint = ntot_host >= 0 ? ntot_host : (c->last_n_hint > 0 ? c->last_n_hint : (int64_t)1 << 40);
     P.n_photons = -1;                               // read dep_off[ndep] on the device
-    const size_t lds_ev = ring_bytes(c->wg);
+    const size_t lds_ev = ring_bytes(c->wg) + emb_bytes(c->wg);
     const int grid = grid_for(c, n_hint, c->wg, c->ppt, lds_ev);
 
     // Events owned by workgroups (one writer per map row, no global atomics on the maps)
@@ -1844,7 +1969,7 @@ int crysp_autotune_events(crysp_ctx *c, const crysp_box *box,
     for (int a = 0; a < n_wg; ++a) {
         for (int b = 0; b < n_ppt; ++b) {
             if (wgs[a] >= 64 && wgs[a] <= 1024 &&
-                ring_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
+                ring_bytes(wgs[a]) + emb_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
                 if (times_ms) times_ms[a * n_ppt + b] = INFINITY;   // does not fit: skipped
                 continue;
             }
@@ -1903,7 +2028,7 @@ static int autotune_sweep(crysp_ctx *c, const int32_t *wgs, int32_t n_wg, const 
             for (int q = 0; q < n_chunk && rc == CRYSP_OK; ++q)
                 for (int r = 0; r < n_refill && rc == CRYSP_OK; ++r, ++idx) {
                     if (wgs[a] >= 64 && wgs[a] <= 1024 &&
-                        ring_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
+                        ring_bytes(wgs[a]) + emb_bytes(wgs[a]) + STATIC_LDS_MARGIN > c->lds_per_block) {
                         if (times_ms) times_ms[idx] = INFINITY;     // does not fit: skipped
                         continue;
                     }
