@@ -284,7 +284,7 @@ def _other_configs(ctx):
     ctx.set_op_counting(False)
     best1, times1 = T.autotune_accumulate(box, op, ro, tb, **kw, wg_sizes=(128, 256, 512),
                                           ppts=(1, 2, 4, 8), chunks=(16, 32, 64, 128, 256),
-                                          refill_mins=(6, 12), reps=2)
+                                          refill_mins=(2, 4, 8, 12), reps=2)
     ctx.set_timing(True)
     ctx.kernel_time(reset=True)
     t0 = time.perf_counter()
@@ -308,7 +308,7 @@ def _other_configs(ctx):
         "end_to_end_photons_per_s": n1 / e2e, "detected_fraction": acc.ndet / n1,
         "roofline": roofline(ops1, ms / n / 1e3), "tail": tail1}
     ctx.set_schedule(*default_schedule)
-    ctx.set_queue(128, 12)
+    ctx.set_queue(256, 4)
 
     dep = E.synthetic_deposits(box, n_events=100_000, seed=11, e_kev=(10.0, 511.0),
                                yield_per_mev=54_000.0)
@@ -316,7 +316,7 @@ def _other_configs(ctx):
     sub = E.Deposits(dep.xyzt[:m], dep.nph[:m], dep.event[:m], dep.e_kev[:m], m)
     best3, times3 = E.autotune_queue(box, op, ro, sub, seed=5, tau_ns=1000.0, tau_rise_ns=20.0,
                                      ctx=ctx, wg_sizes=(128, 256, 512), ppts=(4,),
-                                     chunks=(64, 128, 256), refill_mins=(6, 12, 24), reps=2)
+                                     chunks=(128, 256, 512), refill_mins=(4, 8, 12), reps=2)
     dev = torch.device("cuda", 0)
     darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event))
     nev = dep.n_events
@@ -341,7 +341,7 @@ def _other_configs(ctx):
     ctx.set_timing(False)
     # the tuned knobs belong to this workload; the legs after it run the defaults
     ctx.set_schedule(*default_schedule)
-    ctx.set_queue(128, 12)
+    ctx.set_queue(256, 4)
     nph = dep.nph.astype(np.int64)
     res["batched_uneven_csitl"] = {
         "events": nev, "photons_per_step": r.n_photons,

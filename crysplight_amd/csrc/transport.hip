@@ -1037,8 +1037,8 @@ struct crysp_ctx {
     size_t lds_per_block = 65536;
     int wg = 256;
     int ppt = 4;
-    int chunk = 128;
-    int refill_min = 12;
+    int chunk = 256;
+    int refill_min = 4;
     bool own_events = false;                        // event mode: workgroups own events
     bool timeline = false;                          // record each wave's start and end
     int64_t timeline_waves = 0;                     // waves of the last recorded launch

@@ -283,8 +283,8 @@ def autotune(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_n
 
 def autotune_queue(box: Box, op: OpticalParams, ro, dep: Deposits, *, seed: int, tau_ns: float,
                    tau_rise_ns: float = 0.0, max_bounces: int = 100_000, ctx=None,
-                   wg_sizes=(128, 256, 512), ppts=(4,), chunks=(32, 64, 128, 256),
-                   refill_mins=(6, 12, 24), reps: int = 2):
+                   wg_sizes=(128, 256, 512), ppts=(4,), chunks=(64, 128, 256, 512),
+                   refill_mins=(2, 4, 8, 12), reps: int = 2):
     """Sweep every launch knob -- workgroup size, photons per thread, the work queue's grab
     size (chunk) and refill batch (refill_min) -- on this deposit table (event mode) and keep
     the fastest in the context.  Returns (best (wg, ppt, chunk, refill_min),

@@ -122,8 +122,8 @@ def transport_accumulate(box: Box, op: OpticalParams, ro: Readout, tb: TimeBinni
 def autotune_accumulate(box: Box, op: OpticalParams, ro, tb: TimeBinning, *, n_photons: int,
                         seed: int, pos=None, positions=None, t0s=None, tau_ns: float = 0.0,
                         tau_rise_ns: float = 0.0, max_bounces: int = 100_000, ctx=None,
-                        wg_sizes=(128, 256, 512), ppts=(1, 2, 4, 8), chunks=(32, 64, 128, 256),
-                        refill_mins=(6, 12, 24), reps: int = 2):
+                        wg_sizes=(128, 256, 512), ppts=(1, 2, 4, 8), chunks=(32, 64, 128, 256, 512),
+                        refill_mins=(2, 4, 8, 12), reps: int = 2):
     """Sweep every launch knob (workgroup size, photons per thread, queue chunk, refill
     batch) on this point-source run (accumulate mode) and keep the fastest in the context.
     Returns (best (wg, ppt, chunk, refill_min), times_ms[n_wg, n_ppt, n_chunk, n_refill])."""

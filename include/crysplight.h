@@ -199,7 +199,7 @@ int crysp_get_schedule(crysp_ctx *ctx, int32_t *wg_size, int32_t *photons_per_th
 
 /* Work-queue knobs (no reference counterpart): a wave takes `chunk` consecutive photons
  * per atomic grab of the queue cursor, and hands new photons to its idle lanes once at
- * least refill_min of them are idle (or all are).  Defaults 128 and 12. */
+ * least refill_min of them are idle (or all are).  Defaults 256 and 4. */
 int crysp_set_queue(crysp_ctx *ctx, int32_t chunk, int32_t refill_min);
 
 /* Event-mode work split (no reference counterpart).  enable != 0: when the deposits are

@@ -250,7 +250,7 @@ def test_full_knob_autotuners(ctx, bgo_events):
         assert np.array_equal(acc.counts, r["counts"]) and acc.ndet == r["ndet"]
     finally:
         ctx.set_schedule(*old)
-        ctx.set_queue(128, 12)
+        ctx.set_queue(256, 4)
 
 
 def test_events_queue_parameters_do_not_change_results(ctx, bgo_events):
@@ -268,7 +268,7 @@ def test_events_queue_parameters_do_not_change_results(ctx, bgo_events):
             with pytest.raises(CryspError):
                 ctx.set_queue(*bad)
     finally:
-        ctx.set_queue(128, 12)
+        ctx.set_queue(256, 4)
 
 
 def test_events_bad_event_index_is_an_error(ctx, bgo_events):
