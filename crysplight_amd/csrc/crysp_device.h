@@ -181,10 +181,13 @@ __device__ __forceinline__ void rng_prefetch(Rng &r, uint32_t k0, uint32_t k1)
     if (r.wr - r.rd <= 8u * RING_SLOT) rng_fill(r, k0, k1);
 }
 
-template <bool COUNT>
+// CHECK = false: a draw that cannot outrun the ring.  The top of every step leaves at
+// least eight words (rng_prefetch), so the first eight draws of a step need no check;
+// such a draw has no branch before its LDS read, which the compiler can then move ahead.
+template <bool COUNT, bool CHECK = true>
 __device__ __forceinline__ uint32_t rng_next(Rng &r, const KParams &P, OpCount<COUNT> &oc)
 {
-    if (r.rd == r.wr) rng_fill(r, P.key0, P.key1);
+    if (CHECK && r.rd == r.wr) rng_fill(r, P.key0, P.key1);
     if (COUNT && ((r.rd / RING_SLOT) & 3u) == 0u) oc.add(2);   // a block's first word
     oc.add(1);
     const uint32_t x = *(const uint32_t *)(r.ring + ring_addr(r, r.rd));
@@ -198,10 +201,10 @@ __device__ __forceinline__ float unit_of(uint32_t x)
     return ((float)(x >> 8) + 1.0f) * 5.9604645e-08f;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool CHECK = true>
 __device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> &oc)
 {
-    return unit_of(rng_next<COUNT>(r, P, oc));
+    return unit_of(rng_next<COUNT, CHECK>(r, P, oc));
 }
 
 // ----------------------------------------------------------- transcendentals ----
@@ -408,13 +411,13 @@ constexpr float ABS_BAND = 2.4414062e-04f;           // 2^-12
 // otherwise one draw u reflects when u < R (src/kernel.jl:220-226, 269-274; the UNIFIED
 // tries, src/optics.jl:67-68).  The estimate decides unless u lies within FRESNEL_BAND of
 // it; there the exact R decides.  The outcome is always the exact R's.
-template <bool COUNT>
+template <bool COUNT, bool CHECK = true>
 __device__ __forceinline__ bool fresnel_reflects(Rng &r, const KParams &P, OpCount<COUNT> &oc,
                                                  float q, float n1, float n2, float cosi)
 {
     const float sin2t = fresnel_sin2t(q, cosi);
     if (sin2t >= 1.0f) return true;
-    const float u = randu<COUNT>(r, P, oc);
+    const float u = randu<COUNT, CHECK>(r, P, oc);
     const float re = fresnel_R_est(n1, n2, cosi, sin2t);
     if (u < re - FRESNEL_BAND) return true;
     if (u >= re + FRESNEL_BAND) return false;
@@ -497,20 +500,20 @@ __device__ __forceinline__ void rayleigh(Rng &r, const KParams &P, OpCount<COUNT
 // facet_try: the first half of a try -- the Box-Muller tilt (two draws), the azimuth
 // (one draw) and the facet normal.  False when the try is void (tilt >= 1.2 rad, or the
 // facet faces away from the photon); the caller then moves on to the next try.
-template <bool COUNT>
+template <bool COUNT, bool CHECK = true>
 __device__ __forceinline__ bool facet_try(Rng &r, const KParams &P, OpCount<COUNT> &oc,
                                           float ux, float uy, float uz, int a, float sgn,
                                           float &nfx, float &nfy, float &nfz, float &cosi)
 {
     oc.add(5);
-    const float u1 = randu<COUNT>(r, P, oc);
-    const float u2 = randu<COUNT>(r, P, oc);
+    const float u1 = randu<COUNT, CHECK>(r, P, oc);
+    const float u2 = randu<COUNT, CHECK>(r, P, oc);
     const float g = f_sqrt(-2.0f * f_log(u1)) * f_cos(TWO_PI_F * u2);
     const float alpha = P.sigma_alpha * fabsf(g);
     if (alpha >= 1.2f) return false;
     float sa, ca;
     f_sincos_tilt(alpha, sa, ca);
-    const float phi = TWO_PI_F * randu<COUNT>(r, P, oc);
+    const float phi = TWO_PI_F * randu<COUNT, CHECK>(r, P, oc);
     float sp, cp;
     f_sincos(phi, sp, cp);
     const float b = sa * cp, c = sa * sp;
@@ -676,8 +679,10 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
     // every draw value, pinned by the tests) unless the estimate lies within ABS_BAND of
     // m = min(d, d_ray); there the exact d_abs decides.  The outcome is the exact one's.
     float u_abs = 1.0f, d_ray = inf;
-    if (has<OPT>(P, K_ABS)) u_abs = randu<COUNT>(r, P, oc);
-    if (has<OPT>(P, K_RAY)) d_ray = -P.ray_len * f_log(randu<COUNT>(r, P, oc));
+    // the step's first draws (at most six: absorption, Rayleigh distance, the first
+    // micro-facet try and its Fresnel draw) are within the eight the top of the step holds
+    if (has<OPT>(P, K_ABS)) u_abs = randu<COUNT, false>(r, P, oc);
+    if (has<OPT>(P, K_RAY)) d_ray = -P.ray_len * f_log(randu<COUNT, false>(r, P, oc));
     if (has<OPT>(P, K_ABS)) {
         const float m = d_ray < d ? d_ray : d;
         const float est = P.neg_abs_ln2 * f_log2_est(u_abs);
@@ -732,11 +737,11 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         const bool rough = !readout_face && has<OPT>(P, K_ROUGH);
         float cf = cosi, nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
         bool ok = true;                              // the (first) try reaches Fresnel
-        if (rough) ok = facet_try<COUNT>(r, P, oc, p.ux, p.uy, p.uz, a, sgn, nfx, nfy, nfz, cf);
+        if (rough) ok = facet_try<COUNT, false>(r, P, oc, p.ux, p.uy, p.uz, a, sgn, nfx, nfy, nfz, cf);
         bool refl = false;
         if (ok) {
             oc.add(4);
-            refl = fresnel_reflects<COUNT>(r, P, oc, readout_face ? P.q_coup : P.q_air,
+            refl = fresnel_reflects<COUNT, false>(r, P, oc, readout_face ? P.q_coup : P.q_air,
                                            P.n_crystal, readout_face ? P.n_coupling : 1.0f, cf);
         }
         if (readout_face) {
