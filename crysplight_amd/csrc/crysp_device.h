@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sincos_table.h"          // sin, cos of k pi/128 (f_sincos)
+
 namespace crysp {
 
 enum : int { ST_FLYING = 0, ST_DETECTED = 1, ST_ABS_BULK = 2, ST_ABS_WALL = 3,
@@ -210,10 +212,10 @@ __device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> 
 // ----------------------------------------------------------- transcendentals ----
 // The transport evaluates log only at randu outputs (0, 1] and sin/cos only at
 // phi = 2f0*Float32(pi)*u in (0, 2pi] and at tilts alpha in [0, 1.2).  These versions
-// are written for that domain (any |x| < 2^20 for sin/cos, any positive normal float
+// are written for that domain (x in [0, 2pi] for sin/cos, any positive normal float
 // for log): the float argument is widened to binary64, reduced exactly, expanded in a
 // short series with ~1 ulp binary64 error, and rounded once to binary32 -- the
-// correctly rounded result except when the exact value lies within ~2^-52 relative of
+// correctly rounded result except when the exact value lies within ~2^-51 relative of
 // a binary32 rounding boundary.  The tests compare every one of the 2^24 log and
 // phase arguments against a correctly rounded reference.
 
@@ -278,41 +280,37 @@ __device__ __forceinline__ float f_log(float x)
     return (float)__builtin_fma(ed, 0.6931471803691238, __builtin_fma(ed, 1.9082149292705877e-10, lm));
 }
 
-// sin and cos: x = k pi/2 + r with pi/2 in three parts (33 + 33 + 53 bits, so k * P1
-// and x - k * P1 are exact), |r| <= pi/4, Taylor series to r^17 / r^16 (< 2^-58).
+// sin and cos of x in [0, 2pi (1 + 2^-20)] (the phases 2f0*Float32(pi)*u and the tilts):
+// x = k pi/128 + r with k = rint(128 x / pi) in 0..256 and pi/128 in three parts (33 + 33
+// + 53 bits, so k * P1 and x - k * P1 are exact), |r| <= pi/256.  sin r to r^7 and
+// cos r - 1 to r^6 leave less than 2^-66 relative; with S, C = sin, cos(k pi/128) from a
+// table of correctly rounded binary64 values,
+//   sin x = S + (S (cos r - 1) + C sin r),   cos x = C + (C (cos r - 1) - S sin r),
+// the large term added last.  Over every phase argument the binary64 result is within
+// 2^-51.5 relative of the exact value, which comes no closer than 2^-49.2 to a binary32
+// rounding boundary, so the rounding to binary32 is the correct one; the tests check all
+// 2^24 phases and the tilt domain on the device.  About two thirds of the instructions of
+// a quadrant reduction with r^17 / r^16 series and no quadrant selects.  An index outside
+// the table (an argument outside the domain) is clamped into it.
 __device__ __forceinline__ void f_sincos(float xf, float &sf, float &cf)
 {
     const double x = (double)xf;
-    const double kd = __builtin_rint(x * 0.6366197723675814);
-    double r = __builtin_fma(-kd, 1.5707963267341256, x);
-    r = __builtin_fma(-kd, 6.077100506303966e-11, r);
-    r = __builtin_fma(-kd, 2.0222662487959506e-21, r);
+    const double kd = __builtin_rint(x * 40.74366543152521);       // 128 / pi
+    int k = (int)kd;
+    k = k < 0 ? 0 : (k > 256 ? 256 : k);
+    const double S = SINCOS_TAB[k][0], C = SINCOS_TAB[k][1];
+    double r = __builtin_fma(-kd, 1.5707963267341256 / 64.0, x);
+    r = __builtin_fma(-kd, 6.077100506303966e-11 / 64.0, r);
+    r = __builtin_fma(-kd, 2.0222662487959506e-21 / 64.0, r);
     const double z = r * r;
-    double ps = 2.8114572543455206e-15;                      // 1/17!
-    ps = fma_k(ps, z, -7.647163731819816e-13);
-    ps = fma_k(ps, z, 1.6059043836821613e-10);
-    ps = fma_k(ps, z, -2.505210838544172e-08);
-    ps = fma_k(ps, z, 2.7557319223985893e-06);
-    ps = fma_k(ps, z, -0.0001984126984126984);
-    ps = fma_k(ps, z, 0.008333333333333333);
-    ps = fma_k(ps, z, -0.16666666666666666);
-    const double sn = __builtin_fma(r * z, ps, r);           // sin r
-    double pc = 4.779477332387385e-14;                       // 1/16!
-    pc = fma_k(pc, z, -1.1470745597729725e-11);
-    pc = fma_k(pc, z, 2.08767569878681e-09);
-    pc = fma_k(pc, z, -2.755731922398589e-07);
-    pc = fma_k(pc, z, 2.48015873015873e-05);
-    pc = fma_k(pc, z, -0.001388888888888889);
-    pc = fma_k(pc, z, 0.041666666666666664);
-    pc = __builtin_fma(pc, z, -0.5);
-    const double cs = __builtin_fma(z, pc, 1.0);             // cos r
-    const int q = (int)kd & 3;
-    double sv = (q & 1) ? cs : sn;
-    double cv = (q & 1) ? sn : cs;
-    if (q & 2) sv = -sv;
-    if ((q + 1) & 2) cv = -cv;
-    sf = (float)sv;
-    cf = (float)cv;
+    double q = fma_first(z, -1.0 / 5040.0, 1.0 / 120.0);
+    q = fma_k(q, z, -1.0 / 6.0);
+    const double sn = __builtin_fma(r * z, q, r);                  // sin r
+    double p = fma_first(z, -1.0 / 720.0, 1.0 / 24.0);
+    p = fma_k(p, z, -0.5);
+    const double cm1 = p * z;                                      // cos r - 1
+    sf = (float)(S + __builtin_fma(S, cm1, C * sn));
+    cf = (float)(C + __builtin_fma(C, cm1, -(S * sn)));
 }
 
 // sin and cos of a micro-facet tilt alpha >= 0.  Below 1/8 rad, where alpha = sigma |g|

@@ -243,3 +243,28 @@ def test_every_pointer_taking_entry_point_has_a_signature():
         called |= set(re.findall(r"load\(\)\.(crysp_\w+)\(", f.read_text()))
     missing = sorted(n for n in called if n not in _lib.SIGNATURES)
     assert not missing, missing
+
+
+def test_sincos_table_is_the_generator_output_and_correctly_rounded():
+    """The device's sin/cos table (sin, cos of k pi/128, k = 0..256) is what its generator
+    writes, and every entry is the binary64 rounding of the value computed in x87 extended
+    precision (within half an ulp of binary64 plus the extended precision's own error), with
+    the multiples of pi/2 exact."""
+    import importlib.util
+    from pathlib import Path
+    csrc = Path(__file__).resolve().parents[1] / "crysplight_amd" / "csrc"
+    spec = importlib.util.spec_from_file_location("gen_sincos_table", csrc / "gen_sincos_table.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert (csrc / "sincos_table.h").read_text() == gen.header()
+    rows = np.array(gen.table())
+    assert rows.shape == (257, 2)
+    k = np.arange(257)
+    assert np.array_equal(rows[k % 64 == 0], [[0, 1], [1, 0], [0, -1], [-1, 0], [0, 1]])
+    pi_l = np.longdouble(4) * np.arctan(np.longdouble(1))
+    a = k.astype(np.longdouble) * pi_l / 128
+    for col, f in ((0, np.sin), (1, np.cos)):
+        want = f(a)
+        ok = k % 64 != 0
+        err = np.abs(rows[ok, col].astype(np.longdouble) - want[ok]) / np.spacing(np.abs(rows[ok, col]))
+        assert err.max() < 0.5 + 1e-3, (col, float(err.max()))
