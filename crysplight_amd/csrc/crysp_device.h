@@ -15,7 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "sincos_table.h"          // sin, cos of k pi/128 (f_sincos)
+#include "math_tables.h"           // sin, cos of k pi/128 (f_sincos); 1/c, log c (f_log)
 
 namespace crysp {
 
@@ -219,8 +219,6 @@ __device__ __forceinline__ float randu(Rng &r, const KParams &P, OpCount<COUNT> 
 // a binary32 rounding boundary.  The tests compare every one of the 2^24 log and
 // phase arguments against a correctly rounded reference.
 
-// log(x) = e ln2 + 2 atanh(s),  x = 2^e m,  m in [sqrt(2)/2, sqrt(2)),  s = (m-1)/(m+1),
-// |s| <= 0.1716: the atanh series to s^23 leaves < 2^-55 relative.
 // a * b + c for a binary64 constant c, which the VOP3 fma reads from a scalar register
 // pair.  Written as __builtin_fma, the compiler selects the two-address fmac form, whose
 // addend must be the destination VGPR pair: every Horner step then copied its constant into
@@ -245,39 +243,35 @@ __device__ __forceinline__ double fma_first(double z, double b, double c)
     return r;
 }
 
+// log(x) = e ln2 + log(c) + log1p(r),  x = 2^e m,  m in [sqrt(2)/2, sqrt(2)),  r = m (1/c) - 1
+// (one fma, no division): the low exponent bit and the 7 leading mantissa bits of m pick
+// one of 129 intervals, whose 1/c (rounded) and log(c) = -log(1/c) come from a table; c is
+// 1 for the two intervals next to 1, so log(x) near 1 is log1p(r) alone, accurate relative
+// to itself.  |r| <= 2^-7: log1p to r^8 leaves < 2^-56 relative.  Over every randu value
+// the binary64 result is within 2^-51.5 relative of the exact value, which comes no closer
+// than 2^-47.5 to a binary32 rounding boundary.  Every float argument maps to an index
+// inside the table.
 __device__ __forceinline__ float f_log(float x)
 {
     const uint32_t b = __float_as_uint(x);
     int e = (int)(b >> 23) - 127;
     uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u;
     if (mb > 0x3FB504F3u) { mb -= 0x00800000u; ++e; }      // m >= sqrt(2): halve it
+    const uint32_t i = ((mb >> 16) & 0xFFu) - 0x35u;         // 0..128
+    const double inv = LOG_TAB[i][0], T = LOG_TAB[i][1];
     const double m = (double)__uint_as_float(mb);
-    // s = (m-1)/(m+1) (m-1, m+1 exact) from the hardware reciprocal and two Newton steps:
-    // ~1 ulp of binary64 instead of a correctly rounded division, well inside what the
-    // single rounding to binary32 needs
-    const double num = m - 1.0, den = m + 1.0;
-    double rc = __builtin_amdgcn_rcp(den);
-    double er = __builtin_fma(-den, rc, 1.0);
-    rc = __builtin_fma(rc, er, rc);
-    er = __builtin_fma(-den, rc, 1.0);
-    rc = __builtin_fma(rc, er, rc);
-    const double s = num * rc;
-    const double z = s * s;
-    double p = 0.043478260869565216;                         // 1/23
-    p = fma_k(p, z, 0.047619047619047616);           // 1/21
-    p = fma_k(p, z, 0.05263157894736842);            // 1/19
-    p = fma_k(p, z, 0.058823529411764705);           // 1/17
-    p = fma_k(p, z, 0.06666666666666667);            // 1/15
-    p = fma_k(p, z, 0.07692307692307693);            // 1/13
-    p = fma_k(p, z, 0.09090909090909091);            // 1/11
-    p = fma_k(p, z, 0.1111111111111111);             // 1/9
-    p = fma_k(p, z, 0.14285714285714285);            // 1/7
-    p = fma_k(p, z, 0.2);                            // 1/5
-    p = fma_k(p, z, 0.3333333333333333);             // 1/3
-    const double lm = 2.0 * __builtin_fma(s * z, p, s);      // 2 atanh(s)
+    const double r = __builtin_fma(m, inv, -1.0);
+    const double z = r * r;
+    double p = fma_first(r, -1.0 / 8.0, 1.0 / 7.0);
+    p = fma_k(p, r, -1.0 / 6.0);
+    p = fma_k(p, r, 0.2);
+    p = fma_k(p, r, -0.25);
+    p = fma_k(p, r, 1.0 / 3.0);
+    p = fma_k(p, r, -0.5);
     const double ed = (double)e;
     // ln2 in two parts: ed * LN2_HI is exact for |e| < 2^21
-    return (float)__builtin_fma(ed, 0.6931471803691238, __builtin_fma(ed, 1.9082149292705877e-10, lm));
+    return (float)__builtin_fma(ed, 0.6931471803691238,
+                                __builtin_fma(ed, 1.9082149292705877e-10, T + __builtin_fma(z, p, r)));
 }
 
 // sin and cos of x in [0, 2pi (1 + 2^-20)] (the phases 2f0*Float32(pi)*u and the tilts):

@@ -245,19 +245,22 @@ def test_every_pointer_taking_entry_point_has_a_signature():
     assert not missing, missing
 
 
-def test_sincos_table_is_the_generator_output_and_correctly_rounded():
-    """The device's sin/cos table (sin, cos of k pi/128, k = 0..256) is what its generator
-    writes, and every entry is the binary64 rounding of the value computed in x87 extended
-    precision (within half an ulp of binary64 plus the extended precision's own error), with
-    the multiples of pi/2 exact."""
+def test_math_tables_are_the_generator_output_and_correctly_rounded():
+    """The device's transcendental tables are what their generator writes.  sin, cos of
+    k pi/128 (k = 0..256): every entry the binary64 rounding of the value computed in x87
+    extended precision (within half an ulp plus the extended precision's own error), the
+    multiples of pi/2 exact.  log: for each of the 129 intervals of m in [sqrt(2)/2,
+    sqrt(2)) (low exponent bit and 7 leading mantissa bits 0x35..0xB5), 1/c is the rounded
+    inverse of the interval's midpoint (of 1 next to 1), so that |m (1/c) - 1| <= 2^-7 over
+    the interval, and the second column is -log(1/c) correctly rounded."""
     import importlib.util
     from pathlib import Path
     csrc = Path(__file__).resolve().parents[1] / "crysplight_amd" / "csrc"
-    spec = importlib.util.spec_from_file_location("gen_sincos_table", csrc / "gen_sincos_table.py")
+    spec = importlib.util.spec_from_file_location("gen_math_tables", csrc / "gen_math_tables.py")
     gen = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(gen)
-    assert (csrc / "sincos_table.h").read_text() == gen.header()
-    rows = np.array(gen.table())
+    assert (csrc / "math_tables.h").read_text() == gen.header()
+    rows = np.array(gen.sincos_table())
     assert rows.shape == (257, 2)
     k = np.arange(257)
     assert np.array_equal(rows[k % 64 == 0], [[0, 1], [1, 0], [0, -1], [-1, 0], [0, 1]])
@@ -268,3 +271,17 @@ def test_sincos_table_is_the_generator_output_and_correctly_rounded():
         ok = k % 64 != 0
         err = np.abs(rows[ok, col].astype(np.longdouble) - want[ok]) / np.spacing(np.abs(rows[ok, col]))
         assert err.max() < 0.5 + 1e-3, (col, float(err.max()))
+    lt = np.array(gen.log_table())
+    assert lt.shape == (129, 2)
+    idx = np.arange(0x35, 0xB6)
+    lo = np.where(idx < 128, (128 + (idx & 127)) / 256.0, (128 + (idx & 127)) / 128.0)
+    hi = lo + np.where(idx < 128, 1 / 256.0, 1 / 128.0)
+    lo = np.maximum(lo, np.sqrt(0.5))
+    hi = np.minimum(hi, np.sqrt(2.0))
+    one = (idx == 0x7F) | (idx == 0x80)
+    assert np.array_equal(lt[one], [[1.0, 0.0], [1.0, 0.0]])
+    r = np.maximum(np.abs(lo * lt[:, 0] - 1), np.abs(hi * lt[:, 0] - 1))
+    assert r.max() <= 2.0 ** -7 * (1 + 1e-12), r.max()
+    want = -np.log(lt[~one, 0].astype(np.longdouble))
+    err = np.abs(lt[~one, 1].astype(np.longdouble) - want) / np.spacing(np.abs(lt[~one, 1]))
+    assert err.max() < 0.5 + 1e-3, float(err.max())
