@@ -32,15 +32,31 @@ constexpr int KP = 128;                             // K staged in LDS per phase
 constexpr int LDS_PAD = 8;                          // bf16 elements of row padding
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }   // round to nearest even
+// the bits of to_bf16(x), in the low half of a word
+__device__ __forceinline__ uint32_t bf16_bits(float x)
+{
+    const __bf16 h = to_bf16(x);
+    return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
 __device__ __forceinline__ float from_bf16(__bf16 x) { return (float)x; }
 
-// exact GELU, 0.5 x (1 + erf(x / sqrt 2)), and its derivative
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 x (1 + tanh(u)) = x s with s = sigma(2u),
+// u = sqrt(2/pi) (x + 0.044715 x^3), and its derivative s + x s (1 - s) (2u)'.  The network
+// is trained and evaluated with this activation; it differs from the erf form by less than
+// 5e-4 everywhere.  s comes from the hardware exp and reciprocal (about 16 instructions for
+// both values, against about 60 for erf and its derivative): an exp that overflows gives
+// s = 0, one that underflows s = 1, so the activation is finite everywhere.
+__device__ __forceinline__ float gelu_s(float x)
+{
+    const float v = x * __builtin_fmaf(x * x, 0.07135481627159416f, 1.5957691216057308f);   // 2u
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-v));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.3989422804014327f * __expf(-0.5f * x * x);
-    return cdf + x * pdf;
+    const float s = gelu_s(x);
+    const float dv = __builtin_fmaf(x * x, 0.21406444881478248f, 1.5957691216057308f);      // (2u)'
+    return __builtin_fmaf(x * s * (1.0f - s), dv, s);
 }
 
 enum Epi : int {
@@ -150,22 +166,136 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
     gemm_tile<EPI>(g, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z);
 }
 
-// Several independent split-K products in one launch: product j owns the z range
-// [z0[j], z0[j+1]) (its K slices); blocks beyond its tile grid return at once.
 constexpr int MAXL = 9;                             // layers of the MLP, at most
-struct GroupArgs {
-    int n;
-    GemmArgs a[MAXL];
-    int z0[MAXL + 1];
+
+// ------------------------------------------------------------- weight gradients ----
+// dW_l [out x in_pad] = dZ_l^T H_{l-1} over the batch: A = dZT [out][Mpad] and B = HT (or
+// the packed inputs XT) [in_pad][Mpad], both with the batch row contiguous.  The batch is
+// split into ks slices of kslice rows (ks at most DW_KS whatever the batch, so the
+// reduction stays short); slice s's partial product goes to C + s * out * in_pad and
+// reduce_grads adds the slices in order.  A workgroup computes one DW_M x BN tile of one
+// slice (BM, BN = 128, or 64 where the layer is not a multiple of 128 wide), four waves of
+// BM/2 x BN/2, with K staged
+// DW_KP at a time through two LDS buffers: the next stage's global loads are in flight
+// during the current stage's MFMAs, one barrier per stage.  The workgroups of the first
+// column tile also add up their staged dZ rows: the slice's partial of the bias gradient,
+// in a fixed order (each thread half a row per stage, in k order, stage after stage; the
+// two halves added last).
+constexpr int DW_M = 128, DW_KP = 64, DW_MAXN = 128, DW_KS = 32;
+
+struct DwArgs {
+    const __bf16 *A[MAXL], *B[MAXL];               // dZT, HT (XT for layer 0)
+    int M[MAXL], N[MAXL];                          // out, in_pad (multiples of 64)
+    float *C[MAXL];                                // partial products [ks][M][N]
+    float *db[MAXL];                               // bias partials [ks][M]
+    int t0[MAXL + 1];                              // flat tile ranges of the layers
+    int n;                                         // layers
+    int64_t ld;                                    // Mpad: row stride of A and B
+    int kslice;                                    // batch rows per slice, a multiple of DW_KP
 };
 
-__global__ void __launch_bounds__(256) gemm_grouped(const GroupArgs G)
+template <int BM_, int BN>
+__device__ __forceinline__ void dw_tile(const DwArgs &a, int l, int tile, int slice,
+                                        __bf16 (*sA)[DW_M][DW_KP + LDS_PAD],
+                                        __bf16 (*sB)[DW_MAXN][DW_KP + LDS_PAD])
 {
-    int j = 0;
-    while (j + 1 < G.n && (int)blockIdx.z >= G.z0[j + 1]) ++j;
-    const GemmArgs &g = G.a[j];
-    if ((int)blockIdx.x >= g.N / BN || (int)blockIdx.y >= g.M / BM) return;
-    gemm_tile<EPI_F32>(g, blockIdx.x, blockIdx.y, blockIdx.z - G.z0[j], G.z0[j + 1] - G.z0[j]);
+    constexpr int MT = BM_ / 64, NT = BN / 64;     // 32 x 32 MFMA tiles per wave: MT x NT
+    constexpr int CA = BM_ * (DW_KP / 8) / 256;    // 16-byte chunks per thread per stage
+    constexpr int CB = BN * (DW_KP / 8) / 256;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ntn = a.N[l] / BN;
+    const int m0 = (tile / ntn) * BM_, n0 = (tile % ntn) * BN;
+    const int wm = (wave >> 1) * (BM_ / 2), wn = (wave & 1) * (BN / 2);
+    const int64_t ld = a.ld;
+    const int64_t kb = (int64_t)slice * a.kslice;
+    const int kc = (tid & 7) * 8, r8 = tid >> 3;   // chunk: row r8 + 32 u, k kc .. kc + 7
+    const __bf16 *Ag = a.A[l] + (size_t)m0 * ld + kb + kc;
+    const __bf16 *Bg = a.B[l] + (size_t)n0 * ld + kb + kc;
+    const bool rows = n0 == 0 && a.db[l];
+    bf16x8 ra[CA], rb[CB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < CA; ++u) ra[u] = *(const bf16x8 *)(Ag + (size_t)(r8 + 32 * u) * ld + k0);
+#pragma unroll
+        for (int u = 0; u < CB; ++u) rb[u] = *(const bf16x8 *)(Bg + (size_t)(r8 + 32 * u) * ld + k0);
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < CA; ++u) *(bf16x8 *)&sA[b][r8 + 32 * u][kc] = ra[u];
+#pragma unroll
+        for (int u = 0; u < CB; ++u) *(bf16x8 *)&sB[b][r8 + 32 * u][kc] = rb[u];
+    };
+    f32x16 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    float rsum = 0.0f;                             // bias rows: row tid / 2, half tid % 2
+    const int nst = a.kslice / DW_KP;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
+        if (st + 1 < nst) load((st + 1) * DW_KP);
+#pragma unroll
+        for (int kk = 0; kk < DW_KP; kk += 16) {
+            const int r = lane & 31, kq = kk + 8 * (lane >> 5);
+            bf16x8 fa[MT], fb[NT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i) fa[i] = *(const bf16x8 *)&sA[cur][wm + 32 * i + r][kq];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) fb[j] = *(const bf16x8 *)&sB[cur][wn + 32 * j + r][kq];
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (rows && tid < 2 * BM_) {
+            const __bf16 *q = &sA[cur][tid >> 1][(tid & 1) * (DW_KP / 2)];
+#pragma unroll
+            for (int c = 0; c < DW_KP / 2; c += 8) {
+                const bf16x8 v = *(const bf16x8 *)(q + c);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) rsum += from_bf16(v[e]);
+            }
+        }
+        if (st + 1 < nst) store(cur ^ 1);
+        __syncthreads();
+    }
+    float *C = a.C[l] + (size_t)slice * a.M[l] * a.N[l];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int col = n0 + wn + 32 * j + (lane & 31);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int row = m0 + wm + 32 * i + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                C[(size_t)row * a.N[l] + col] = acc[i][j][e];
+            }
+        }
+    if (rows) {
+        const float other = __shfl_xor(rsum, 1);
+        if ((tid & 1) == 0 && tid < 2 * BM_) a.db[l][(size_t)slice * a.M[l] + m0 + (tid >> 1)] = rsum + other;
+    }
+}
+
+__global__ void __launch_bounds__(256) dw_grouped(const DwArgs a)
+{
+    __shared__ __bf16 sA[2][DW_M][DW_KP + LDS_PAD];
+    __shared__ __bf16 sB[2][DW_MAXN][DW_KP + LDS_PAD];
+    int l = 0;
+    while (l + 1 < a.n && (int)blockIdx.x >= a.t0[l + 1]) ++l;
+    const int tile = blockIdx.x - a.t0[l];
+    const bool wm = a.M[l] % 128 == 0, wn = a.N[l] % 128 == 0;
+    if (wm && wn) dw_tile<128, 128>(a, l, tile, blockIdx.y, sA, sB);
+    else if (wm) dw_tile<128, 64>(a, l, tile, blockIdx.y, sA, sB);
+    else if (wn) dw_tile<64, 128>(a, l, tile, blockIdx.y, sA, sB);
+    else dw_tile<64, 64>(a, l, tile, blockIdx.y, sA, sB);
 }
 
 // ---------------------------------------------------------------- small kernels ----
@@ -186,14 +316,16 @@ __global__ void pack_inputs(const float *X, int M, int Mpad, int n_in, int kin_p
 
 // ------------------------------------------------------------ the training step ----
 // A step is four launches: mlp_rows (inputs, forward, loss, the backward data path and the
-// bias-gradient partials, one workgroup per 32 rows), gemm_grouped (the weight gradients of
-// every layer as split-K products), reduce_grads (the partials summed in a fixed order) and
-// adam_refresh (Adam, the bf16 weight copies).  Every sum runs in a fixed order, so a
+// loss partials, one workgroup per 64 rows), dw_grouped (the weight gradients of every layer
+// as split-K products, with the bias row sums), reduce_grads (the partials summed in a fixed
+// order) and adam_refresh (Adam, the bf16 weight copies).  Every sum runs in a fixed order, so a
 // training run is reproducible.
 
-constexpr int RB = 32;                              // rows per mlp_rows workgroup
-constexpr int RT = 512;                             // its threads: 8 waves, one 32-column block each
 constexpr int AMAX = 256;                           // widest layer (hidden, n_out, kin_pad)
+constexpr int CBLK = AMAX / 32;                     // 32-column blocks of the widest layer
+constexpr int RB = 64;                              // rows per mlp_rows workgroup: two groups of 32
+constexpr int RT = 64 * CBLK * (RB / 32);           // its threads: 16 waves, wave w on column
+                                                    // block w % CBLK of row group w / CBLK
 constexpr int KB = 128;                             // K staged in LDS per phase (mlp_rows)
 
 struct RowArgs {
@@ -205,7 +337,6 @@ struct RowArgs {
     __bf16 *HT[MAXL];                               // hidden layers: activation, transposed [out][Mpad]
     __bf16 *dZT[MAXL];                              // gradient of layer l's pre-activation, [out][Mpad]
     __bf16 *XT;                                     // packed inputs, transposed [kin_pad][Mpad]
-    float *bpart[MAXL];                             // [n_wg][out] bias-gradient partials
     float *lpart;                                   // [n_wg] loss partials
     const float *Xraw, *T, *W, *shift, *scale;
     const float *R0;                                // rate offsets [M][n_pois] or null
@@ -278,8 +409,9 @@ struct BStream {
 };
 
 // out[RB x N] = A[RB x K] * B^T for the next GEMM of the stream (phases q .. q + np - 1),
-// A resident in LDS.  Wave w owns the 32-column block w, lane element i (0..15) of it at
-// (row, col).  pre(i, row, col, u, v, r) loads what the epilogue of an element needs into
+// A resident in LDS.  Wave w owns the 32 x 32 block of column block w % CBLK and row group
+// w / CBLK, lane element i (0..15) of it at (row, col); the two row groups share every
+// staged slice of B.  pre(i, row, col, u, v, r) loads what the epilogue of an element needs into
 // u, v, r before the K loop, so those loads complete under the MFMAs;
 // epi(i, row, col, acc, u, v, r) stores the element.
 template <class Pre, class Epi>
@@ -288,13 +420,14 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
                                           Pre pre, Epi epi)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool mine = wave < N / 32;
-    const int col = wave * 32 + (lane & 31);
+    const int cb = wave % CBLK, rbase = (wave / CBLK) * 32;
+    const bool mine = cb < N / 32;
+    const int col = cb * 32 + (lane & 31);
     float pu[16], pv[16], pr[16];
     if (mine) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            pre(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i], pr[i]);
+            pre(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i], pr[i]);
     }
     f32x16 acc;
 #pragma unroll
@@ -307,8 +440,8 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
         if (mine) {
             for (int kk = 0; kk < kp; kk += 16) {
                 const int r = lane & 31, kq = kk + 8 * (lane >> 5);
-                const bf16x8 a = *(const bf16x8 *)&A[r][k0 + kq];
-                const bf16x8 b = *(const bf16x8 *)&sB[wave * 32 + r][kq];
+                const bf16x8 a = *(const bf16x8 *)&A[rbase + r][k0 + kq];
+                const bf16x8 b = *(const bf16x8 *)&sB[cb * 32 + r][kq];
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
             }
         }
@@ -317,7 +450,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     if (mine) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            epi(i, 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i], pr[i]);
+            epi(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i], pr[i]);
     }
 }
 
@@ -335,17 +468,6 @@ __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD]
     }
 }
 
-// column sums of the RB rows of act (bf16) into out[c], c < N, rows in order
-__device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD], int N, float *out)
-{
-    for (int c = threadIdx.x; c < N; c += RT) {
-        float s = 0.0f;
-#pragma unroll 8
-        for (int r = 0; r < RB; ++r) s += from_bf16(act[r][c]);
-        out[c] = s;
-    }
-}
-
 // One workgroup per RB rows of the batch:
 //   pack (x - shift) * scale to bf16;
 //   forward: hidden layers Z = A W^T + b, H = gelu(Z) (bf16) and gelu'(Z) (fp32, for the
@@ -354,7 +476,7 @@ __device__ __forceinline__ void rows_colsum(const __bf16 (*act)[AMAX + LDS_PAD],
 //     columns c <  n_pois: Poisson NLL of a log-rate, w (e^y - t y), gradient w (e^y - t);
 //     columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t);
 //   backward data path: dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1});
-//   bias-gradient partials (column sums of every dZ_l over the RB rows) and the loss partial.
+//   the loss partial (the bias gradients come from dw_grouped, which reads every dZ_l).
 // Padded rows (>= M) carry zero inputs and a zero gradient.
 // TRAIN = false is the inference pass: the forward layers only, the last one's output (with
 // its bias) stored to a.Y, nothing else written.
@@ -388,7 +510,8 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
     float lsum = 0.0f;
     // gelu'(Z) of the hidden layers, kept in registers: the backward product of layer l + 1
     // has the forward layer l's shape, and every element stays with the lane that made it
-    float gpr[L - 1][16];
+    // (as bf16, two to a register: 24 registers for three hidden layers instead of 48)
+    uint32_t gpr[L - 1][8];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         const int N = a.dout[l];
@@ -399,7 +522,11 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
                       [&](int, int row, int col, float &u, float &, float &) { u = bias[col]; },
                       [&](int i, int row, int col, float v, float u, float, float) {
                           v += u;
-                          if (TRAIN) gpr[l < L - 1 ? l : 0][i] = gelu_grad(v);
+                          if (TRAIN) {
+                              const uint32_t hb = bf16_bits(gelu_grad(v));
+                              uint32_t &w = gpr[l < L - 1 ? l : 0][i >> 1];
+                              w = (i & 1) ? (w | (hb << 16)) : hb;
+                          }
                           out[row][col] = to_bf16(gelu(v));
                       });
         } else if (!TRAIN) {
@@ -415,18 +542,18 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
             rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int row, int col, float &t, float &w, float &r) {
-                          const int rg = r0 + row;
-                          const size_t o = (size_t)rg * N + col;
-                          t = rg < M ? T[o] : 0.0f;
-                          w = rg < M ? (W ? W[o] : 1.0f) : 0.0f;
-                          r = (R0 && rg < M && col < n_pois) ? R0[(size_t)rg * n_pois + col] : 0.0f;
-                      },
-                      [&](int, int row, int col, float y, float t, float w, float r) {
+                      [&](int, int, int, float &, float &, float &) {},
+                      [&](int, int row, int col, float y, float, float, float) {
+                          // targets, weights and rate offsets are read here (four waves per
+                          // SIMD hide the latency; prefetched, they would not fit in registers)
                           y += bias[col];
                           const int rg = r0 + row;
                           float g = 0.0f;
                           if (rg < M) {
+                              const size_t o = (size_t)rg * N + col;
+                              const float t = T[o];
+                              const float w = W ? W[o] : 1.0f;
+                              const float r = (R0 && col < n_pois) ? R0[(size_t)rg * n_pois + col] : 0.0f;
                               if (col < n_pois) {
                                   // rate = e^y + r (r: a known part of the rate, 0 without
                                   // offsets): w (rate - t log rate), gradient w (1 - t/rate) e^y
@@ -453,7 +580,6 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         if (TRAIN) rows_store_T(act[cur], N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
     }
     if (!TRAIN) return;
-    rows_colsum(act[cur], a.dout[L - 1], a.bpart[L - 1] + (size_t)blockIdx.x * a.dout[L - 1]);
 
 #pragma unroll
     for (int l = L - 1; l >= 1; --l) {
@@ -462,12 +588,12 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         rows_gemm(act[cur], a.dout[l], N, bs, q, sB,
                   [&](int, int, int, float &, float &, float &) {},
                   [&](int i, int row, int col, float v, float, float, float) {
-                      out[row][col] = to_bf16(v * gpr[l - 1][i]);
+                      const uint32_t w = gpr[l - 1][i >> 1];
+                      out[row][col] = to_bf16(v * __uint_as_float((i & 1) ? (w & 0xFFFF0000u) : (w << 16)));
                   });
         __syncthreads();
         cur ^= 1;
         rows_store_T(act[cur], N, a.dZT[l - 1], a.Mpad, r0);
-        rows_colsum(act[cur], N, a.bpart[l - 1] + (size_t)blockIdx.x * N);
     }
 
     // loss partial of the workgroup, in a fixed order
@@ -487,8 +613,9 @@ struct RedArgs {
     int L, n_wg;
     const float *part[MAXL];                        // split-K partials of dW_l, slice stride dout * dpad
     int ks[MAXL], din[MAXL], dpad[MAXL], dout[MAXL];
-    const float *bpart[MAXL];                       // [n_wg][dout]
+    const float *db[MAXL];                          // bias partials of the slices [ks][dout]
     int64_t woff[MAXL], boff[MAXL], cum[MAXL + 1];  // cum: outputs (dout * din + dout) before layer l
+    int64_t bcum[MAXL + 1];                         // biases (dout) before layer l
     float *g;
     const float *lpart;
     float *loss;                                    // loss[ctr[0]] with loss_by_batch, else loss[0]
@@ -513,8 +640,8 @@ __device__ __forceinline__ float sum_strided(const float *q, int n, int64_t stri
 }
 
 // The gradient: every weight is the in-order sum of its K-slice partials, every bias the
-// in-order sum of the per-workgroup column sums; the loss is the in-order sum of the
-// workgroup partials.  With ctr (graph replays) the batch index and the Adam step count
+// in-order sum of the slices' row sums; the loss is the in-order sum of the workgroup
+// partials.  With ctr (graph replays) the batch index and the Adam step count
 // advance here, after the step's last reader of the batch index and before Adam.
 __global__ void __launch_bounds__(256) reduce_grads(const RedArgs a)
 {
@@ -539,9 +666,15 @@ __global__ void __launch_bounds__(256) reduce_grads(const RedArgs a)
             }
             for (; u < a.ks[l]; ++u) s += q[(size_t)u * slice];
             a.g[a.woff[l] + j] = s;
-        } else {
-            a.g[a.boff[l] + (j - nw)] = sum_strided(a.bpart[l] + (j - nw), a.n_wg, a.dout[l]);
         }
+    }
+    // biases: the slices' partials in order
+    for (int64_t bi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; bi < a.bcum[a.L];
+         bi += (int64_t)gridDim.x * blockDim.x) {
+        int l = 0;
+        while (bi >= a.bcum[l + 1]) ++l;
+        const int o = (int)(bi - a.bcum[l]);
+        a.g[a.boff[l] + o] = sum_strided(a.db[l] + o, a.ks[l], a.dout[l]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const float s = sum_strided(a.lpart, a.n_wg, 1);
@@ -1046,7 +1179,7 @@ struct crysp_mlp {
     std::vector<__bf16 *> dZ;                       // [cap][out] per layer
     float *part = nullptr;                          // split-K partial weight gradients, all layers
     std::vector<size_t> part_off;                   // per layer, floats
-    float *bpart = nullptr;                         // bias-gradient partials [n_wg][out], all layers
+    float *bpart = nullptr;                         // the slices' bias row sums [ks][out], all layers
     std::vector<size_t> bpart_off;                  // per layer, floats
     float *lpart = nullptr;                         // loss partials [n_wg]
     float *loss = nullptr;
@@ -1064,17 +1197,12 @@ struct crysp_mlp {
     double flops_fwd_per_row = 0.0;
 };
 
-// K slices for a weight-gradient product (output M x N, reduction over the batch K): the
-// output alone is only a few 64x64 tiles, so K is split until about SPLIT_TARGET_WG
-// workgroups run per layer (all layers share one launch), keeping each slice a multiple of
-// BK and at least two BK steps.
-constexpr int SPLIT_TARGET_WG = 256;
-static int weight_grad_slices(int M, int N, int K)
+// Batch slices of the weight-gradient products: DW_KS, or fewer where Mpad / DW_KP has
+// fewer power-of-two factors, so that every slice is a whole number of DW_KP stages.
+static int dw_slices(int Mpad)
 {
-    const int tiles = (M / BM) * (N / BN);
     int ks = 1;
-    while (tiles * ks * 2 <= SPLIT_TARGET_WG && K % (ks * 2 * BK) == 0 && K / (ks * 2) >= 2 * BK)
-        ks *= 2;
+    while (ks * 2 <= DW_KS && (Mpad / DW_KP) % (ks * 2) == 0) ks *= 2;
     return ks;
 }
 
@@ -1108,14 +1236,11 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
         SCHK(hipMalloc(&n->Z[l], (size_t)cap * n->dout[l] * 4));
         SCHK(hipMalloc(&n->H[l], (size_t)cap * n->dout[l] * 2));
         SCHK(hipMalloc(&n->dZ[l], (size_t)cap * n->dout[l] * 2));
-        // the most slices any batch can get: tiles * slices stays within SPLIT_TARGET_WG
-        const int tiles = (n->dout[l] / BM) * (n->dpad[l] / BN);
-        int ks_max = 1;
-        while (tiles * ks_max * 2 <= SPLIT_TARGET_WG) ks_max *= 2;
+        // the slices' partial weight gradients and bias row sums (at most DW_KS slices)
         n->part_off[l] = part;
-        part += (size_t)ks_max * n->dout[l] * n->dpad[l];
+        part += (size_t)DW_KS * n->dout[l] * n->dpad[l];
         n->bpart_off[l] = bpart;
-        bpart += (size_t)n_wg * n->dout[l];
+        bpart += (size_t)DW_KS * n->dout[l];
     }
     fre(n->part); fre(n->bpart); fre(n->lpart);
     n->part = nullptr; n->bpart = nullptr; n->lpart = nullptr;
@@ -1169,7 +1294,7 @@ static int mlp_infer_dev(crysp_mlp *n, const float *Xdev, int64_t M, float *Ydev
 // The gradient of the weighted loss on Mpad rows (M real) into n->g (fp32 flat) and the loss
 // into loss_out[0], or loss_out[ctr[0]] with loss_by_batch.  With ctr (graph replays),
 // ctr[0] is the batch index whose rows are read, and ctr advances.  Three launches:
-// mlp_rows, gemm_grouped (dW of every layer), reduce_grads.
+// mlp_rows, dw_grouped (dW and the bias row sums of every layer), reduce_grads.
 static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const float *W,
                         const float *R0, int M, int Mpad, double wsum, float *loss_out,
                         bool loss_by_batch, int64_t *ctr)
@@ -1185,7 +1310,6 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
         ra.Wt[l] = n->Wt[l]; ra.Wn[l] = n->Wn[l]; ra.bias[l] = n->p + n->boff[l];
         ra.HT[l] = n->H[l];
         ra.dZT[l] = l + 1 == L ? n->dY : n->dZ[l];
-        ra.bpart[l] = n->bpart + n->bpart_off[l];
     }
     ra.XT = n->X; ra.lpart = n->lpart;
     ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.R0 = R0; ra.shift = n->shift; ra.scale = n->scale;
@@ -1203,39 +1327,34 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     }                                               // the layer loops unroll (gelu' in registers)
 
     // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad], both operands stored
-    // transposed (k = the batch row contiguous), every layer in one launch, as split-K
-    // partial products
-    GroupArgs ga{};
-    ga.n = L;
+    // transposed (k = the batch row contiguous), every layer in one launch, as the partial
+    // products of ks batch slices (with the bias row sums)
+    const int ks = dw_slices(Mpad);
+    DwArgs da{};
+    da.n = L; da.ld = Mpad; da.kslice = Mpad / ks;
     RedArgs rd{};
     rd.L = L; rd.n_wg = n_wg;
-    int z = 0;
+    int tiles = 0;
     int64_t cum = 0;
     for (int l = 0; l < L; ++l) {
         const int kin = n->dpad[l], nout = n->dout[l];
-        const int ks = weight_grad_slices(nout, kin, Mpad);
-        GemmArgs &a = ga.a[l];
-        a.M = nout; a.N = kin; a.K = Mpad;
-        a.A = ra.dZT[l]; a.lda = Mpad; a.a_t = 0;
-        a.B = l == 0 ? ra.XT : ra.HT[l - 1]; a.ldb = Mpad; a.b_t = 1;
-        a.C = n->part + n->part_off[l]; a.ldc = kin; a.c_slice = (int64_t)nout * kin;
-        ga.z0[l] = z;
-        z += ks;
-        rd.part[l] = a.C; rd.ks[l] = ks;
+        da.A[l] = ra.dZT[l];
+        da.B[l] = l == 0 ? ra.XT : ra.HT[l - 1];
+        da.M[l] = nout; da.N[l] = kin;
+        da.C[l] = n->part + n->part_off[l];
+        da.db[l] = n->bpart + n->bpart_off[l];
+        da.t0[l] = tiles;
+        tiles += (nout / (nout % 128 == 0 ? 128 : 64)) * (kin / (kin % 128 == 0 ? 128 : 64));
+        rd.part[l] = da.C[l]; rd.ks[l] = ks; rd.db[l] = da.db[l];
         rd.din[l] = n->din[l]; rd.dpad[l] = kin; rd.dout[l] = nout;
-        rd.bpart[l] = ra.bpart[l];
         rd.woff[l] = (int64_t)n->woff[l]; rd.boff[l] = (int64_t)n->boff[l];
         rd.cum[l] = cum;
         cum += (int64_t)nout * n->din[l] + nout;
+        rd.bcum[l + 1] = rd.bcum[l] + nout;
     }
-    ga.z0[L] = z;
+    da.t0[L] = tiles;
     rd.cum[L] = cum;
-    int gx = 1, gy = 1;
-    for (int l = 0; l < L; ++l) {
-        gx = std::max(gx, ga.a[l].N / BN);
-        gy = std::max(gy, ga.a[l].M / BM);
-    }
-    hipLaunchKernelGGL(gemm_grouped, dim3(gx, gy, z), dim3(256), 0, n->stream, ga);
+    hipLaunchKernelGGL(dw_grouped, dim3(tiles, ks), dim3(256), 0, n->stream, da);
 
     rd.g = n->g; rd.lpart = n->lpart; rd.loss = loss_out; rd.loss_by_batch = loss_by_batch; rd.ctr = ctr;
     hipLaunchKernelGGL(reduce_grads, dim3((unsigned)std::min<int64_t>(1024, (cum + 255) / 256)), dim3(256),

@@ -1,7 +1,8 @@
 """Float64 restatement of the surrogate MLP (test infrastructure only).
 
 Forward: x' = (x - shift) * scale; h_l = GELU(h_{l-1} W_l^T + b_l) for the hidden layers
-(exact GELU, 0.5 x (1 + erf(x / sqrt 2))); y = h W_L^T + b_L.
+(GELU in its tanh form, 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))), the activation the
+device network uses); y = h W_L^T + b_L.
 Loss: L = sum W (y - t)^2 / sum W.  Gradients by reverse mode.  Parameters use the
 library's flat layout: per layer W [out][in] row-major, then b [out].
 
@@ -14,7 +15,6 @@ from __future__ import annotations
 import math
 
 import numpy as np
-from scipy.special import erf
 
 
 def layer_shapes(n_in=4, hidden=256, n_hidden_layers=3, n_out=128):
@@ -42,12 +42,16 @@ def round_bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+_C = math.sqrt(2.0 / math.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+    return 0.5 * z * (1.0 + np.tanh(_C * (z + 0.044715 * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+    t = np.tanh(_C * (z + 0.044715 * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * _C * (1.0 + 3 * 0.044715 * z * z)
 
 
 def forward(p, X, shapes, shift=None, scale=None, bf16=False):
