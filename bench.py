@@ -56,8 +56,10 @@ def main() -> int:
                     help="skip the surrogate training / validation / event comparison")
     ap.add_argument("--surrogate-events", type=int, default=1_000_000)
     ap.add_argument("--surrogate-samples", type=int, default=1_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=300,
+    ap.add_argument("--surrogate-epochs", type=int, default=1200,
                     help="training epochs per crystal (fewer only for profiling runs)")
+    ap.add_argument("--surrogate-batch", type=int, default=65536,
+                    help="training batch (rows per Adam step)")
     ap.add_argument("--surrogate-val-points", type=int, default=10_000)
     ap.add_argument("--surrogate-near-readout", type=float, default=0.2,
                     help="fraction of training points drawn within 2 mm of the readout face")
@@ -401,6 +403,7 @@ def _surrogate(args, ctx):
         sur, info = SE.train_for_box(box, op, ro, tau_ns=tau, tau_rise_ns=rise,
                                      n_samples=args.surrogate_samples, photons=10_000, ctx=ctx,
                                      seed=11 + int(t), epochs=args.surrogate_epochs,
+                                     batch=args.surrogate_batch,
                                      near_readout_frac=args.surrogate_near_readout)
         surs.append(sur)
         train[f"{t:g}mm"] = info

@@ -30,6 +30,7 @@ that interpolates a baked grid of the network.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -488,6 +489,7 @@ class LightSurrogate:
         Wb = torch.empty((n, W_d.shape[1]), dtype=W_d.dtype, device=dev)
         Rb = None if R_d is None else torch.empty((n, R_d.shape[1]), dtype=R_d.dtype, device=dev)
         losses = []
+        self.fit_step_s = 0.0                           # time in the training steps alone
         for ep in range(epochs):
             lr_ep = lr * (lr_final / lr) ** (ep / max(1, epochs - 1))
             perm = torch.randperm(ts.X.shape[0], generator=g, device=dev)[:n]
@@ -497,8 +499,10 @@ class LightSurrogate:
             if Rb is not None:
                 torch.index_select(R_d, 0, perm, out=Rb)
             torch.cuda.synchronize()
+            t0 = time.perf_counter()
             l = self.net.train_batches(Xb, Tb, Wb, n // batch, batch, lr=lr_ep,
-                                       mean_row_weight=mrw, R0=Rb)
+                                       mean_row_weight=mrw, R0=Rb)      # synchronizes
+            self.fit_step_s += time.perf_counter() - t0
             losses.append(l)
             if verbose:
                 print(f"epoch {ep}: lr {lr_ep:.2e} loss {l.mean():.5f}", flush=True)

@@ -14,6 +14,7 @@ compare_events         : gamma events in several crystal thicknesses through (a)
 """
 from __future__ import annotations
 
+import math
 import time
 
 import numpy as np
@@ -31,7 +32,7 @@ def _sync():
 
 
 def train_for_box(box, op, ro, *, tau_ns, tau_rise_ns=0.0, n_samples=1_000_000,
-                  photons=10_000, epochs=300, batch=8192, seed=11, log_depth_mm=1.0, fold=True,
+                  photons=10_000, epochs=1200, batch=65536, seed=11, log_depth_mm=1.0, fold=True,
                   direct=True, images=True, near_readout_frac=0.0, near_readout_mm=2.0, ctx=None):
     ctx = ctx or _lib.default_context()
     sur = S.LightSurrogate(box, op, ro, tau_ns=tau_ns, tau_rise_ns=tau_rise_ns, seed=seed,
@@ -41,17 +42,25 @@ def train_for_box(box, op, ro, *, tau_ns, tau_rise_ns=0.0, n_samples=1_000_000,
                       near_readout_mm=near_readout_mm, ctx=ctx)
     _sync()
     t1 = time.perf_counter()
-    losses = sur.fit(ts, epochs=epochs, batch=batch, lr=3e-3, lr_final=5e-5, seed=seed)
+    # Adam's step size grows as the square root of the batch beyond 8192 rows (3e-3 there),
+    # decaying 60-fold over the epochs; batch 65536 at 1200 epochs fits more accurately than
+    # 8192 at 300 with the same data, and its steps run at about twice the MFMA rate
+    lr = 3e-3 * math.sqrt(max(1.0, batch / 8192))
+    losses = sur.fit(ts, epochs=epochs, batch=batch, lr=lr, lr_final=lr / 60, seed=seed)
     _sync()
     t2 = time.perf_counter()
     info = {"n_samples": n_samples, "photons_per_sample": photons, "generate_s": t1 - t0,
-            "fit_s": t2 - t1, "epochs": epochs, "batch": batch, "adam_steps": int(losses.size),
+            "fit_s": t2 - t1, "epochs": epochs, "batch": batch, "lr": lr, "adam_steps": int(losses.size),
             "symmetry_folded": sur.fold, "direct_light_term": sur.direct,
             "mirror_image_terms": sur.images,
             "near_readout_samples": [near_readout_frac, near_readout_mm],
             "final_loss": float(losses[-min(50, losses.size):].mean()),
             "train_flops": float(3.0 * sur.net.flops_per_row * (losses.size * batch))}
     info["train_tflops_per_s"] = info["train_flops"] / info["fit_s"] / 1e12
+    # the Adam steps alone (the fit also reshuffles the training set every epoch)
+    info["step_us"] = sur.fit_step_s / max(1, losses.size) * 1e6
+    info["step_tflops_per_s"] = info["train_flops"] / sur.fit_step_s / 1e12
+    info["step_frac_bf16_dense_peak"] = info["step_tflops_per_s"] / 2500.0
     return sur, info
 
 
