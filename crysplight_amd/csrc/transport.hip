@@ -891,15 +891,21 @@ __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *
                               int32_t *npe, int64_t nev, int nxy, const unsigned int *flag,
                               int gated)
 {
+    // one wave per event: lane c takes cells c, c + 64, ... (coalesced), and npe is the sum
+    // of the row's counts (integers: the order of the additions does not matter)
     if (gated && !(*flag & FLAG_UNSORTED)) return;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nev;
-         e += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; e < nev; e += nw) {
         int32_t s = 0;
-        for (int c = 0; c < nxy; ++c) {
-            s += maps[e * nxy + c];
-            tmin[e * nxy + c] = key_time(keys[e * nxy + c]);
+        for (int c = lane; c < nxy; c += 64) {
+            const int64_t i = e * nxy + c;
+            s += maps[i];
+            tmin[i] = key_time(keys[i]);
         }
-        npe[e] = s;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        if (lane == 0) npe[e] = s;
     }
 }
 
@@ -909,8 +915,10 @@ __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *
 __global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, int64_t ndep,
                                                       int64_t *off, unsigned int *flag)
 {
-    __shared__ int64_t part[1024];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    // each thread sums a contiguous run of deposits; the runs' exclusive prefix comes from an
+    // inclusive scan within each wave (shuffles) and one over the wave totals
+    __shared__ int64_t wsum[16];
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
     const int64_t per = (ndep + nt - 1) / nt;
     const int64_t lo = std::min<int64_t>(ndep, (int64_t)tid * per);
     const int64_t hi = std::min<int64_t>(ndep, lo + per);
@@ -922,15 +930,28 @@ __global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, int64_
         s += v > 0 ? v : 0;
     }
     if (bad) atomicOr(flag, bad);
-    part[tid] = s;
+    int64_t x = s;                                  // inclusive prefix within the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    if (tid == 0) {
-        int64_t acc = 0;
-        for (int i = 0; i < nt; ++i) { const int64_t v = part[i]; part[i] = acc; acc += v; }
-        off[ndep] = acc;
+    if (wave == 0) {
+        const int nwv = nt >> 6;
+        int64_t t = lane < nwv ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int64_t y = __shfl_up(t, d);
+            if (lane >= d) t += y;
+        }
+        if (lane < nwv) wsum[lane] = t;             // inclusive prefix over the waves
     }
     __syncthreads();
-    int64_t acc = part[tid];
+    const int64_t base = wave ? wsum[wave - 1] : 0;
+    if (tid == nt - 1) off[ndep] = base + x;        // the photon total
+    int64_t acc = base + x - s;
     for (int64_t i = lo; i < hi; ++i) { off[i] = acc; const int32_t v = nph[i]; acc += v > 0 ? v : 0; }
 }
 
@@ -1807,7 +1828,7 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                 launch_optics_variant<M_EVENTS>(P, B, dim3(grid), dim3(c->wg), lds_ev, c->stream);
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev1, c->stream));
         }
-        const dim3 ev_grid((unsigned)std::min<int64_t>(1024, (nev + 255) / 256));
+        const dim3 ev_grid((unsigned)std::min<int64_t>(4096, (nev + 3) / 4));     // a wave per event
         if (nev > 0 && run_old)
             hipLaunchKernelGGL(finish_events, ev_grid, dim3(256), 0, c->stream, (const int32_t *)dmaps,
                                (const uint32_t *)dkeys, (float *)dtmin, (int32_t *)dnpe, nev, nxy,
