@@ -1386,6 +1386,16 @@ static void mlp_refresh(crysp_mlp *n)
     }
 }
 
+// A copy to or from the network's buffers on its own stream, finished on return.  The
+// stream is non-blocking, so a plain hipMemcpy (the null stream) is not ordered after the
+// kernels and copies queued on it; every host access to the network's memory goes here.
+static int mlp_copy(crysp_mlp *n, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    SCHK(hipMemcpyAsync(dst, src, bytes, kind, n->stream));
+    SCHK(hipStreamSynchronize(n->stream));
+    return CRYSP_OK;
+}
+
 static bool is_dev(const void *p)
 {
     if (!p) return false;
@@ -1452,7 +1462,7 @@ crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_
         ok = ok && hipMalloc(&b, (size_t)n->dpad[l] * n->dout[l] * 2) == hipSuccess;
         n->Wt.push_back(a); n->Wn.push_back(b);
     }
-    if (!ok) { sfail(CRYSP_ENOMEM, "MLP allocation failed"); delete n; return nullptr; }
+    if (!ok) { sfail(CRYSP_ENOMEM, "MLP allocation failed"); crysp_mlp_destroy(n); return nullptr; }
     // He-uniform initialization on the host from a Philox stream (domain 5), zero biases
     std::vector<float> h(n->n_params, 0.0f);
     uint64_t ctr = 0;
@@ -1476,20 +1486,25 @@ crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_
             h[n->woff[l] + i] = (float)((2.0 * uni() - 1.0) * bound);
     }
     std::vector<float> sh(32, 0.0f), sc(32, 1.0f);
-    if (hipMemcpy(n->p, h.data(), n->n_params * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(n->m, 0, n->n_params * 4) != hipSuccess ||
-        hipMemset(n->v, 0, n->n_params * 4) != hipSuccess ||
-        hipMemset(n->g, 0, n->n_params * 4) != hipSuccess ||
-        hipMemcpy(n->shift, sh.data(), 128, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(n->scale, sc.data(), 128, hipMemcpyHostToDevice) != hipSuccess) {
+    // all on the network's stream (mlp_refresh then writes every element of Wt and Wn)
+    const hipStream_t s = n->stream;
+    if (hipMemcpyAsync(n->p, h.data(), n->n_params * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(n->m, 0, n->n_params * 4, s) != hipSuccess ||
+        hipMemsetAsync(n->v, 0, n->n_params * 4, s) != hipSuccess ||
+        hipMemsetAsync(n->g, 0, n->n_params * 4, s) != hipSuccess ||
+        hipMemcpyAsync(n->shift, sh.data(), 128, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(n->scale, sc.data(), 128, hipMemcpyHostToDevice, s) != hipSuccess) {
+        (void)hipStreamSynchronize(s);
         sfail(CRYSP_EHIP, "MLP initialization copies failed");
-        delete n;
+        crysp_mlp_destroy(n);
         return nullptr;
     }
-    for (int l = 0; l < n->n_layers; ++l)
-        (void)hipMemset(n->Wn[l], 0, (size_t)n->dpad[l] * n->dout[l] * 2);
     mlp_refresh(n);
-    (void)hipStreamSynchronize(n->stream);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        sfail(CRYSP_EHIP, "MLP initialization failed");
+        crysp_mlp_destroy(n);
+        return nullptr;
+    }
     return n;
 }
 
@@ -1519,14 +1534,15 @@ int64_t crysp_mlp_num_params(const crysp_mlp *n) { return n ? (int64_t)n->n_para
 int crysp_mlp_get_params(crysp_mlp *n, float *out)
 {
     if (!n || !out) return sfail(CRYSP_EINVAL, "null argument");
-    SCHK(hipMemcpy(out, n->p, n->n_params * 4, hipMemcpyDeviceToHost));
-    return CRYSP_OK;
+    SCHK(hipSetDevice(n->device));
+    return mlp_copy(n, out, n->p, n->n_params * 4, hipMemcpyDeviceToHost);
 }
 
 int crysp_mlp_set_params(crysp_mlp *n, const float *in)
 {
     if (!n || !in) return sfail(CRYSP_EINVAL, "null argument");
-    SCHK(hipMemcpy(n->p, in, n->n_params * 4, hipMemcpyHostToDevice));
+    SCHK(hipSetDevice(n->device));
+    SCHK(hipMemcpyAsync(n->p, in, n->n_params * 4, hipMemcpyHostToDevice, n->stream));
     mlp_refresh(n);
     SCHK(hipStreamSynchronize(n->stream));
     return CRYSP_OK;
@@ -1536,9 +1552,9 @@ int crysp_mlp_set_params(crysp_mlp *n, const float *in)
 int crysp_mlp_set_input_norm(crysp_mlp *n, const float *shift, const float *scale)
 {
     if (!n || !shift || !scale) return sfail(CRYSP_EINVAL, "null argument");
-    SCHK(hipMemcpy(n->shift, shift, n->n_in * 4, hipMemcpyHostToDevice));
-    SCHK(hipMemcpy(n->scale, scale, n->n_in * 4, hipMemcpyHostToDevice));
-    return CRYSP_OK;
+    SCHK(hipSetDevice(n->device));
+    int rc = mlp_copy(n, n->shift, shift, n->n_in * 4, hipMemcpyHostToDevice);
+    return rc ? rc : mlp_copy(n, n->scale, scale, n->n_in * 4, hipMemcpyHostToDevice);
 }
 
 int crysp_mlp_set_poisson_outputs(crysp_mlp *n, int32_t n_pois)
@@ -1593,10 +1609,17 @@ int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float
         return rc;
     double wsum = (double)M * n->n_out;
     if (W) {
-        std::vector<float> hw((size_t)M * n->n_out);
-        SCHK(hipMemcpy(hw.data(), sw.d, hw.size() * 4, hipMemcpyDeviceToHost));
+        // host weights are summed where they are; device ones are read back on the
+        // network's stream, after the staging copies queued there
+        std::vector<float> hw;
+        const float *w = W;
+        if (!sw.own) {
+            hw.resize((size_t)M * n->n_out);
+            if ((rc = mlp_copy(n, hw.data(), sw.d, hw.size() * 4, hipMemcpyDeviceToHost))) return rc;
+            w = hw.data();
+        }
         wsum = 0.0;
-        for (float x : hw) wsum += x;
+        for (size_t i = 0; i < (size_t)M * n->n_out; ++i) wsum += w[i];
         if (!(wsum > 0.0)) return sfail(CRYSP_EINVAL, "loss weights sum to zero");
     }
     if ((rc = mlp_grad_dev(n, (const float *)sx.d, (const float *)st.d, (const float *)sw.d,
@@ -1942,8 +1965,9 @@ crysp_surrogate *crysp_surrogate_create(crysp_mlp *net, const crysp_box *box,
     s->tab.images = cfg->images && op->air_gap;
     if (hipMalloc(&s->dH, H.size() * 4) != hipSuccess ||
         hipMalloc(&s->dinv, inv.size() * 4) != hipSuccess ||
-        hipMemcpy(s->dH, H.data(), H.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->dinv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpyAsync(s->dH, H.data(), H.size() * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->dinv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
         (void)hipGetLastError();
         crysp_surrogate_destroy(s);
         sfail(CRYSP_ENOMEM, "surrogate tables: allocation failed");

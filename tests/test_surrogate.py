@@ -169,27 +169,38 @@ def test_gemm_layout_against_numpy(a_t, b_t):
 
 
 @gpu
-def test_mlp_gradient_against_fp64_oracle():
+@pytest.mark.parametrize("hidden,n_hidden,n_out,rows", [
+    (256, 3, 128, 256),        # the surrogate's network; 4 batch slices of one K stage
+    (256, 3, 128, 4096),       # 32 slices of two stages
+    (192, 2, 64, 1088),        # 64-row weight-gradient tiles; one slice of 17 stages
+    (64, 1, 192, 200),         # padded rows (200 of 256), 64-wide hidden layer
+])
+def test_mlp_gradient_against_fp64_oracle(hidden, n_hidden, n_out, rows):
+    """Forward outputs, loss and every layer's weight and bias gradient of the device step
+    against the FP64 oracle on the same bf16-rounded weights and inputs, over network shapes
+    and batch sizes that take each tile shape of the weight-gradient kernel and several
+    batch-slice layouts (the bias gradients are the slices' dZ row sums there)."""
     from crysplight_amd.surrogate import MLP
-    net = MLP(4, 256, 3, 128, seed=3)
+    net = MLP(4, hidden, n_hidden, n_out, seed=3)
     rng = np.random.default_rng(2)
     shift = np.array([12.8, 12.8, 10.0, 260.0], np.float32)
     scale = np.array([1 / 12.8, 1 / 12.8, 1 / 10.0, 1 / 250.0], np.float32)
     net.set_input_norm(shift, scale)
-    X = np.column_stack([rng.uniform(0, 25.6, 256), rng.uniform(0, 25.6, 256),
-                         rng.uniform(0, 20, 256), rng.uniform(10, 511, 256)]).astype(np.float32)
-    T = rng.normal(0, 1, (256, 128)).astype(np.float32)
-    W = rng.uniform(0, 1, (256, 128)).astype(np.float32)
+    X = np.column_stack([rng.uniform(0, 25.6, rows), rng.uniform(0, 25.6, rows),
+                         rng.uniform(0, 20, rows), rng.uniform(10, 511, rows)]).astype(np.float32)
+    T = rng.normal(0, 1, (rows, n_out)).astype(np.float32)
+    W = rng.uniform(0, 1, (rows, n_out)).astype(np.float32)
     p = net.get_params()
     y_dev = net.forward(X)
     y_ref, _ = M.forward(p, X, net.shapes, shift, scale, bf16=True)
     assert np.linalg.norm(y_dev - y_ref) / np.linalg.norm(y_ref) < 2e-2
-    net.set_poisson_outputs(64)
-    T[:, :64] = np.abs(T[:, :64]) * 0.01
-    R0 = rng.uniform(0.0, 0.01, (256, 64)).astype(np.float32)
+    n_pois = n_out // 2
+    net.set_poisson_outputs(n_pois)
+    T[:, :n_pois] = np.abs(T[:, :n_pois]) * 0.01
+    R0 = rng.uniform(0.0, 0.01, (rows, n_pois)).astype(np.float32)
     for r0 in (None, R0):
         loss, g = net.gradient(X, T, W, R0=r0)
-        L, gr = M.loss_and_grad(p, X, T, W, net.shapes, shift, scale, bf16=True, n_pois=64, R0=r0)
+        L, gr = M.loss_and_grad(p, X, T, W, net.shapes, shift, scale, bf16=True, n_pois=n_pois, R0=r0)
         assert abs(loss - L) / abs(L) < 1e-2, (r0 is None, loss, L)
         o = 0
         for (n_o, n_i) in net.shapes:
