@@ -13,6 +13,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "math_tables.h"           // sin, cos of k pi/128 (f_sincos); 1/c, log c (f_log)
@@ -75,6 +76,23 @@ __device__ __forceinline__ bool has(const KParams &P, uint32_t bit)
 {
     return OPT == OPT_GENERIC ? (P.flags & bit) != 0u : (OPT & bit) != 0u;
 }
+
+// A field of the running transport kernel's arguments, read from the kernel argument
+// segment at each use (KParams first, as the kernel's first argument).  Code that runs
+// rarely (a detection, a refill, an emission, the session bookkeeping) reads its parameters
+// this way, so they hold no scalar registers across the photon loop: there the compiler
+// would keep each one live and, short of scalar registers, spill it to a vector lane, which
+// costs a v_readlane (a vector instruction) at every use.  Only for code that
+// transport_kernel runs.
+template <typename T>
+__device__ __forceinline__ T karg_at(size_t off)
+{
+    const __attribute__((address_space(4))) char *kp =
+        (const __attribute__((address_space(4))) char *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    return *(const __attribute__((address_space(4))) T *)(kp + off);
+}
+#define KP(f) karg_at<decltype(KParams::f)>(offsetof(KParams, f))
 
 // ------------------------------------------------------------------ Philox ----
 // Philox4x32-10 (src/philox.jl:10-28): ten rounds, the key bumped between rounds.
@@ -568,8 +586,9 @@ __device__ __forceinline__ void emit(Rng &r, const KParams &P, OpCount<COUNT> &o
     f_sincos(phi, sp, cp);
     ux = rr * cp; uy = rr * sp; uz = z;
     t = t0;
-    if (P.tau_d > 0.0f) t += -P.tau_d * f_log(randu<COUNT>(r, P, oc));
-    if (P.tau_r > 0.0f) t += -P.tau_r * f_log(randu<COUNT>(r, P, oc));
+    const float tau_d = KP(tau_d), tau_r = KP(tau_r);
+    if (tau_d > 0.0f) t += -tau_d * f_log(randu<COUNT>(r, P, oc));
+    if (tau_r > 0.0f) t += -tau_r * f_log(randu<COUNT>(r, P, oc));
 }
 
 // Emission in batches.  A wave computes the emissions of the next (up to) 64 photons it
@@ -600,11 +619,12 @@ __device__ __forceinline__ void emit_batch_slot(const KParams &P, uint64_t pid, 
     float sp, cp;
     f_sincos(phi, sp, cp);
     uint32_t A = w.z, Bw = w.w;
-    if (P.tau_d > 0.0f) {
-        A = __float_as_uint(-P.tau_d * f_log(unit_of(w.z)));
-        if (P.tau_r > 0.0f) Bw = __float_as_uint(-P.tau_r * f_log(unit_of(w.w)));
-    } else if (P.tau_r > 0.0f) {
-        A = __float_as_uint(-P.tau_r * f_log(unit_of(w.z)));
+    const float tau_d = KP(tau_d), tau_r = KP(tau_r);
+    if (tau_d > 0.0f) {
+        A = __float_as_uint(-tau_d * f_log(unit_of(w.z)));
+        if (tau_r > 0.0f) Bw = __float_as_uint(-tau_r * f_log(unit_of(w.w)));
+    } else if (tau_r > 0.0f) {
+        A = __float_as_uint(-tau_r * f_log(unit_of(w.z)));
     }
     buf[s] = __float_as_uint(rr * cp);
     buf[EMB_SLOTS + s] = __float_as_uint(rr * sp);
@@ -621,7 +641,7 @@ __device__ __forceinline__ void emit_take(Rng &r, const KParams &P, uint64_t pid
     uy = __uint_as_float(buf[EMB_SLOTS + s]);
     uz = __uint_as_float(buf[2 * EMB_SLOTS + s]);
     const uint32_t A = buf[3 * EMB_SLOTS + s], Bw = buf[4 * EMB_SLOTS + s];
-    const bool dec = P.tau_d > 0.0f, ris = P.tau_r > 0.0f;
+    const bool dec = KP(tau_d) > 0.0f, ris = KP(tau_r) > 0.0f;
     const uint32_t m = 2u + (dec ? 1u : 0u) + (ris ? 1u : 0u);
     rng_init(r, pid);
     r.blk = 1u;
@@ -767,11 +787,13 @@ __device__ __forceinline__ int step(Photon &p, Rng &r, const KParams &P, OpCount
         const bool pass = randu<COUNT>(r, P, oc) <= thr;
         if (kind == 1) {
             if (!pass) return ST_ABS_SIPM;
-            int ii = 1 + (int)(p.x / P.pitch);
-            int jj = 1 + (int)(p.y / P.pitch);
-            ii = ii < P.nx ? ii : P.nx;
-            jj = jj < P.ny ? jj : P.ny;
-            const int l = (jj - 1) * P.nx + ii;
+            const float pitch = KP(pitch);
+            const int nx = KP(nx), ny = KP(ny);
+            int ii = 1 + (int)(p.x / pitch);
+            int jj = 1 + (int)(p.y / pitch);
+            ii = ii < nx ? ii : nx;
+            jj = jj < ny ? jj : ny;
+            const int l = (jj - 1) * nx + ii;
             lin = face == F_FRONT ? -l : l;
             oc.add(8);
             return ST_DETECTED;

@@ -159,22 +159,11 @@ __device__ __forceinline__ void lds_set_u64(uint32_t *p, uint64_t v)
     p[1] = (uint32_t)(v >> 32);
 }
 
-// A field of the transport kernel's arguments (KParams, then KBuffers, as the kernel
-// argument segment lays them out) read again at each use.  The session code's pointers and
-// sizes are used rarely; read this way they hold no scalar registers across the photon
-// loop, where the compiler would otherwise keep every one of them live.  Only for code
-// that runs inside transport_kernel.
+// A field of the transport kernel's KBuffers (its second argument, right after KParams in
+// the kernel argument segment) read again at each use, as KP reads KParams: for code that
+// runs rarely inside transport_kernel.
 static_assert(sizeof(KParams) % alignof(KBuffers) == 0, "KBuffers follows KParams directly");
-template <typename T>
-__device__ __forceinline__ T karg_at(size_t off)
-{
-    const __attribute__((address_space(4))) char *kp =
-        (const __attribute__((address_space(4))) char *)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(kp));
-    return *(const __attribute__((address_space(4))) T *)(kp + off);
-}
 #define KB(f) karg_at<decltype(KBuffers::f)>(sizeof(KParams) + offsetof(KBuffers, f))
-#define KP(f) karg_at<decltype(KParams::f)>(offsetof(KParams, f))
 
 // Session q's sums written out by one wave (every lane; q wave-uniform), the slot freed.
 __device__ void sess_flush(uint32_t *S, int q, int nxy, const KBuffers &B)
@@ -479,9 +468,9 @@ transport_kernel(const KParams P, const KBuffers B)
         float x0, y0, z0, t0;
         if (MODE == M_EVENTS) {
             const int64_t d = (int64_t)lds_u64(b + EMB_HDR + EH_EB_DEP) + b[5 * EMB_SLOTS + slot];
-            const float4 q = B.dep[d];
+            const float4 q = KB(dep)[d];
             x0 = q.x; y0 = q.y; z0 = q.z; t0 = q.w;
-            ev = B.dep_event[d];
+            ev = KB(dep_event)[d];
             t_start = t0;
             if (MULTIBOX) {
                 const int32_t bi = B.dep_box[d];
@@ -494,7 +483,7 @@ transport_kernel(const KParams P, const KBuffers B)
             t0 = B.pt ? B.pt[j] : 0.0f;
         }
         p.x = x0; p.y = y0; p.z = z0; p.bounces = 0;
-        emit_take(rng, P, P.pid0 + (uint64_t)k + 1ull, b, slot, t0, p.ux, p.uy, p.uz, p.t);
+        emit_take(rng, P, KP(pid0) + (uint64_t)k + 1ull, b, slot, t0, p.ux, p.uy, p.uz, p.t);
         nscat = 0;
         alive = true;
     };
@@ -541,14 +530,15 @@ transport_kernel(const KParams P, const KBuffers B)
                 uint32_t left = lds_uni(h + EH_RES_LEFT);
                 if (left == 0u) {                           // the next chunk of the queue
                     uint64_t b = 0;
+                    const uint32_t chunk = (uint32_t)KP(chunk);
                     if (__lane_id() == __builtin_amdgcn_readfirstlane(__lane_id()))
-                        b = atomicAdd(B.cursor, (unsigned long long)P.chunk);
+                        b = atomicAdd(KB(cursor), (unsigned long long)chunk);
                     b = first_lane_u64(b);
                     if (b >= N) exhausted = true;
                     else {
                         rn = b;
-                        left = N - b < (uint64_t)P.chunk ? (uint32_t)(N - b) : (uint32_t)P.chunk;
-                        if (MODE == M_EVENTS) rd = find_deposit(B.dep_off, P.ndep, (int64_t)b);
+                        left = N - b < (uint64_t)chunk ? (uint32_t)(N - b) : chunk;
+                        if (MODE == M_EVENTS) rd = find_deposit(KB(dep_off), KP(ndep), (int64_t)b);
                     }
                 }
                 const uint32_t nb = left < EMB_SLOTS ? left : EMB_SLOTS;
@@ -559,11 +549,12 @@ transport_kernel(const KParams P, const KBuffers B)
                 uint32_t last = 0u;
                 if (__lane_id() < nb) {
                     const uint64_t kk = rn + __lane_id();
-                    emit_batch_slot(P, P.pid0 + kk + 1ull, EMB_BUF, __lane_id());
+                    emit_batch_slot(P, KP(pid0) + kk + 1ull, EMB_BUF, __lane_id());
                     if (MODE == M_EVENTS) {
                         // the deposit of photon kk: at or after the chunk's, a few at most
+                        const int64_t *off = KB(dep_off);
                         int64_t d = rd;
-                        while (B.dep_off[d + 1] <= (int64_t)kk) ++d;
+                        while (off[d + 1] <= (int64_t)kk) ++d;
                         last = (uint32_t)(d - rd);
                         EMB_BUF[5 * EMB_SLOTS + __lane_id()] = last;
                     }
@@ -708,15 +699,15 @@ transport_kernel(const KParams P, const KBuffers B)
                         atomicMin(&m[nxy + l - 1], time_key(p.t));
                     }
                 } else {
-                    if (st == ST_DETECTED && (uint32_t)ev < (uint32_t)P.nev) {
+                    if (st == ST_DETECTED && (uint32_t)ev < (uint32_t)KP(nev)) {
                         const int l = lin < 0 ? -lin : lin;
-                        const int64_t cell = (int64_t)(l - 1) + (int64_t)nxy * ev;
-                        atomicAdd(&B.maps[cell], 1);
-                        atomicMin(&B.ev_tkeys[cell], time_key(p.t));
-                        if (B.ev_tsum) {
+                        const int64_t cell = (int64_t)(l - 1) + (int64_t)(KP(nx) * KP(ny)) * ev;
+                        atomicAdd(&KB(maps)[cell], 1);
+                        atomicMin(&KB(ev_tkeys)[cell], time_key(p.t));
+                        unsigned long long *tsum = KB(ev_tsum);
+                        if (tsum) {
                             const float dt = p.t - t_start;
-                            atomicAdd(&B.ev_tsum[cell],
-                                      (unsigned long long)(dt > 0.0f ? dt * 1024.0f : 0.0f));
+                            atomicAdd(&tsum[cell], (unsigned long long)(dt > 0.0f ? dt * 1024.0f : 0.0f));
                         }
                     }
                 }
@@ -909,72 +900,120 @@ __global__ void finish_events(const int32_t *maps, const uint32_t *keys, float *
     }
 }
 
-// exclusive scan of the per-deposit photon counts (device-resident inputs); one
-// workgroup, each thread a contiguous slice.  off[ndep] = total.  flag bit 1: a negative
-// count (taken as 0).
-__global__ void __launch_bounds__(1024) scan_deposits(const int32_t *nph, int64_t ndep,
-                                                      int64_t *off, unsigned int *flag)
-{
-    // each thread sums a contiguous run of deposits; the runs' exclusive prefix comes from an
-    // inclusive scan within each wave (shuffles) and one over the wave totals
-    __shared__ int64_t wsum[16];
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t per = (ndep + nt - 1) / nt;
-    const int64_t lo = std::min<int64_t>(ndep, (int64_t)tid * per);
-    const int64_t hi = std::min<int64_t>(ndep, lo + per);
-    int64_t s = 0;
-    unsigned bad = 0u;
-    for (int64_t i = lo; i < hi; ++i) {
-        const int32_t v = nph[i];
-        if (v < 0) bad |= 1u;
-        s += v > 0 ? v : 0;
-    }
-    if (bad) atomicOr(flag, bad);
-    int64_t x = s;                                  // inclusive prefix within the wave
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int64_t y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    if (wave == 0) {
-        const int nwv = nt >> 6;
-        int64_t t = lane < nwv ? wsum[lane] : 0;
-#pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            const int64_t y = __shfl_up(t, d);
-            if (lane >= d) t += y;
-        }
-        if (lane < nwv) wsum[lane] = t;             // inclusive prefix over the waves
-    }
-    __syncthreads();
-    const int64_t base = wave ? wsum[wave - 1] : 0;
-    if (tid == nt - 1) off[ndep] = base + x;        // the photon total
-    int64_t acc = base + x - s;
-    for (int64_t i = lo; i < hi; ++i) { off[i] = acc; const int32_t v = nph[i]; acc += v > 0 ? v : 0; }
-}
+// The set-up of an event launch, in one kernel.  Workgroup 0 owns the flag word and stores
+// it once, at its end: it clears the queue cursor (and the op counters), scans the
+// device-resident photon counts into offsets (off[ndep] = the total; bit 1: a negative
+// count, taken as 0) and checks device-resident index tables (bit 2: an event outside
+// [0, nev); bit 4: a crystal outside [0, nbox); bit 8, not an error: events out of order,
+// so they are not contiguous photon ranges).  The transport kernel guards both table reads
+// itself; the bits turn a bad table into an error for the caller.  The other workgroups
+// clear the output cells (counts 0, first-time keys +inf) when the launch fills them here,
+// and one further range (the time sums).
+struct PrepArgs {
+    unsigned long long *cursor;
+    unsigned int *flag;                             // 4 words, all rewritten
+    const int32_t *nph;                             // device counts, or null (offsets given)
+    int64_t *off;                                   // [ndep + 1]
+    const int32_t *ev, *box;                        // tables to check, or null
+    int64_t ndep, nev;
+    int32_t nbox;
+    unsigned long long *opc;                        // op counters [10], or null
+    uint32_t *maps, *keys;                          // [cells] each, or null
+    int64_t cells;
+    uint32_t *zero;                                 // [nzero] to clear, or null
+    int64_t nzero;
+};
 
-// Index checks of a deposit table on the device, for tables that live there: flag bit 2
-// an event index outside [0, nev), bit 4 a crystal index outside [0, nbox) (dep_box may be
-// null).  The transport kernel guards both reads itself; these bits turn a bad table into
-// an error for the caller.  Bit 8 (not an error) marks a table whose events are not in
-// non-decreasing order: its events are not contiguous photon ranges.
-__global__ void check_deposits(const int32_t *ev, const int32_t *box, int64_t ndep, int64_t nev,
-                               int32_t nbox, unsigned int *flag)
+// deposits per thread and pass of workgroup 0: its loads are issued together, and each
+// thread's run stays in registers between the sum and the offsets
+constexpr int PREP_RUN = 16;
+
+__global__ void __launch_bounds__(1024) prepare_events(const PrepArgs a)
 {
-    unsigned bad = 0u;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ndep;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t e = ev[i];
-        if (e < 0 || (int64_t)e >= nev) bad |= 2u;
-        if (i > 0 && ev[i - 1] > e) bad |= FLAG_UNSORTED;
-        if (box) {
-            const int32_t b = box[i];
-            if (b < 0 || b >= nbox) bad |= 4u;
-        }
+    const int tid = threadIdx.x;
+    if (blockIdx.x > 0) {
+        const int64_t nt = (int64_t)(gridDim.x - 1) * blockDim.x;
+        const int64_t i0 = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
+        if (a.maps)
+            for (int64_t i = i0; i < a.cells; i += nt) { a.maps[i] = 0u; a.keys[i] = KEY_INF; }
+        if (a.zero)
+            for (int64_t i = i0; i < a.nzero; i += nt) a.zero[i] = 0u;
+        return;
     }
-    if (bad) atomicOr(flag, bad);
+    __shared__ int64_t wsum[16];
+    __shared__ unsigned int sbad;
+    if (tid == 0) sbad = 0u;
+    __syncthreads();                                // before any thread's bits reach it
+    const int lane = tid & 63, wave = tid >> 6, nwv = blockDim.x >> 6;
+    unsigned bad = 0u;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < a.ndep; base += (int64_t)blockDim.x * PREP_RUN) {
+        const int64_t lo = base + (int64_t)tid * PREP_RUN;
+        // every load of a run is predicated (no early exit), so they are issued together;
+        // past the table's end an event reads as the largest index and a crystal as 0
+        if (a.ev) {
+            int32_t e[PREP_RUN];
+#pragma unroll
+            for (int j = 0; j < PREP_RUN; ++j) e[j] = lo + j < a.ndep ? a.ev[lo + j] : INT32_MAX;
+            int32_t prev = lo > 0 && lo <= a.ndep ? a.ev[lo - 1] : INT32_MIN;
+#pragma unroll
+            for (int j = 0; j < PREP_RUN; ++j) {
+                if (lo + j < a.ndep && (e[j] < 0 || (int64_t)e[j] >= a.nev)) bad |= 2u;
+                if (prev > e[j]) bad |= FLAG_UNSORTED;
+                prev = e[j];
+            }
+        }
+        if (a.box) {
+            int32_t bx[PREP_RUN];
+#pragma unroll
+            for (int j = 0; j < PREP_RUN; ++j) bx[j] = lo + j < a.ndep ? a.box[lo + j] : 0;
+#pragma unroll
+            for (int j = 0; j < PREP_RUN; ++j)
+                if (bx[j] < 0 || bx[j] >= a.nbox) bad |= 4u;
+        }
+        if (!a.nph) continue;
+        int32_t v[PREP_RUN];
+        int64_t s = 0;
+#pragma unroll
+        for (int j = 0; j < PREP_RUN; ++j) {
+            int32_t x = lo + j < a.ndep ? a.nph[lo + j] : 0;
+            if (x < 0) { bad |= 1u; x = 0; }
+            v[j] = x;
+            s += x;
+        }
+        // the runs' exclusive prefix: an inclusive scan within each wave, then one over the
+        // wave totals
+        int64_t x = s;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            int64_t t = lane < nwv ? wsum[lane] : 0;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int64_t y = __shfl_up(t, d);
+                if (lane >= d) t += y;
+            }
+            if (lane < nwv) wsum[lane] = t;         // inclusive prefix over the waves
+        }
+        __syncthreads();
+        int64_t acc = carry + (wave ? wsum[wave - 1] : 0) + x - s;
+        carry += wsum[nwv - 1];
+#pragma unroll
+        for (int j = 0; j < PREP_RUN; ++j)
+            if (lo + j < a.ndep) { a.off[lo + j] = acc; acc += v[j]; }
+        __syncthreads();                            // the next pass rewrites wsum
+    }
+    if (a.nph && tid == 0) a.off[a.ndep] = carry;   // the photon total
+    if (a.opc && tid < 10) a.opc[tid] = 0ull;
+    if (tid == 0) *a.cursor = 0ull;
+    if (bad) atomicOr(&sbad, bad);
+    __syncthreads();
+    if (tid < 4) a.flag[tid] = tid == 0 ? sbad : 0u;
 }
 
 __global__ void debug_draws_kernel(uint64_t seed, uint64_t pid0, int64_t n_pids, int n_draws,
@@ -1772,28 +1811,31 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
     // timing events bracket the transport kernel when launched eagerly, and the whole
     // graph (a few-microsecond fill/scan/finish around the kernel) when replayed
     auto enqueue = [&](bool capturing) -> int {
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)cur, 0u, 2);
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)dflag, 0u, 4);
-        if (nph_d)
-            hipLaunchKernelGGL(scan_deposits, dim3(1), dim3(1024), 0, c->stream, nph_d, ndep,
-                               (int64_t *)doff, (unsigned int *)dflag);
+        // cursor, flags, offsets, index checks and the output fills in one launch;
         // device-resident index tables are checked on the device (host ones were checked
         // before any launch)
+        PrepArgs pa{};
+        pa.cursor = (unsigned long long *)cur;
+        pa.flag = (unsigned int *)dflag;
+        pa.nph = nph_d;
+        pa.off = (int64_t *)doff;
         const int32_t *box_chk = box_dev ? B.dep_box : nullptr;
-        if (ndep > 0 && (ev_dev || box_chk))
-            hipLaunchKernelGGL(check_deposits, dim3((unsigned)std::min<int64_t>(1024, (ndep + 255) / 256)),
-                               dim3(256), 0, c->stream, B.dep_event, box_chk, ndep, nev, nbox,
-                               (unsigned int *)dflag);
+        pa.ev = ndep > 0 && (ev_dev || box_chk) ? B.dep_event : nullptr;
+        pa.box = ndep > 0 ? box_chk : nullptr;
+        pa.ndep = ndep; pa.nev = nev; pa.nbox = nbox;
+        pa.opc = c->count_ops ? B.opcounts : nullptr;
+        if (run_old && !sess) {
+            pa.maps = (uint32_t *)dmaps; pa.keys = (uint32_t *)dkeys; pa.cells = (int64_t)cells;
+        }
+        if (tsum_dev) { pa.zero = (uint32_t *)tsum_dev; pa.nzero = (int64_t)cells * 2; }
+        const int64_t nfill = std::max<int64_t>(pa.maps ? pa.cells : 0, pa.zero ? pa.nzero : 0);
+        const unsigned prep_grid = 1u + (unsigned)std::min<int64_t>(512, (nfill + 4095) / 4096);
+        hipLaunchKernelGGL(prepare_events, dim3(prep_grid), dim3(1024), 0, c->stream, pa);
         if (run_old && sess) {                      // only if the device picks this path
             hipLaunchKernelGGL(fill_u32_gated, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps,
                                0u, (int64_t)cells, (const unsigned int *)dflag, 1);
             hipLaunchKernelGGL(fill_u32_gated, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
                                KEY_INF, (int64_t)cells, (const unsigned int *)dflag, 1);
-        } else if (run_old) {
-            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dmaps, 0u,
-                               (int64_t)cells);
-            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)dkeys,
-                               KEY_INF, (int64_t)cells);
         }
         if (sess) {
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, B.sess_ctr, 0u, 4);
@@ -1807,11 +1849,6 @@ static int events_impl(crysp_ctx *c, const crysp_box *box, const crysp_optical_p
                                (const unsigned long long *)B.ev_next, B.ev_end, nev,
                                (uint32_t *)B.ev_order, (const unsigned int *)dflag, (int)ev_dev);
         }
-        if (tsum_dev)
-            hipLaunchKernelGGL(fill_u32, dim3(1024), dim3(256), 0, c->stream, (uint32_t *)tsum_dev, 0u,
-                               (int64_t)cells * 2);
-        if (c->count_ops)
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, c->stream, (uint32_t *)B.opcounts, 0u, 20);
         if ((rc = timeline_setup(c, tl_waves))) return rc;
         if (ndep > 0) {
             if (c->timing && !capturing) HIPCHK(hipEventRecord(c->ev0, c->stream));

@@ -384,6 +384,59 @@ def test_device_tables_with_bad_indices_raise(ctx, bgo_events):
     _check(res, _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU), nev)
 
 
+def test_device_tables_longer_than_one_scan_pass(ctx):
+    """The launch set-up scans device-resident photon counts in passes of 16384 deposits
+    with a carry between passes, and checks the index tables in the same passes.  40000
+    deposits (three passes, the last partial; counts 0 to about 12, so zeros sit inside the
+    runs) of three deposits per event: maps, first times, npe and the photon total against
+    the oracle; a negative count in the third pass is an error; events out of order only in
+    the second pass make the owned-event path stand aside (the atomic path's maps)."""
+    import torch
+    n = 40000
+    single = E.synthetic_deposits(K.C2_BOX, n_events=n, seed=23, e_kev=(0.2, 1.5),
+                                  yield_per_mev=8500.0)
+    ev = (np.arange(n) // 3).astype(np.int32)
+    nev = int(ev[-1]) + 1
+    dep = E.Deposits(single.xyzt, single.nph, ev, single.e_kev, nev)
+    assert (dep.nph == 0).any() and dep.nph.max() > 5
+    ref = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep, 7, K.C2_TAU)
+    dev = torch.device("cuda", 0)
+    out = (torch.zeros((nev, 8, 8), dtype=torch.int32, device=dev),
+           torch.zeros((nev, 8, 8), dtype=torch.float32, device=dev),
+           torch.zeros(nev, dtype=torch.int32, device=dev))
+    darr = tuple(torch.from_numpy(a).to(dev) for a in (dep.xyzt, dep.nph, dep.event))
+    print(f"{n} deposits, {nev} events, {int(dep.nph.sum())} photons", flush=True)
+    for rep in range(2):
+        r = E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                               device_arrays=darr, out=out)
+        assert r.n_photons == int(dep.nph.sum()), rep
+        assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref["maps"]), rep
+        assert np.array_equal(out[1].cpu().numpy().reshape(nev, -1), ref["tmin"]), rep
+        assert np.array_equal(out[2].cpu().numpy(), ref["npe"]), rep
+    nph_bad = dep.nph.copy()
+    nph_bad[2 * 16384 + 5] = -1
+    darr_bad = (darr[0], torch.from_numpy(nph_bad).to(dev), darr[2])
+    with pytest.raises(CryspError, match="negative"):
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=darr_bad, out=out)
+    # swap two deposits' event indices (and with them nothing else: the oracle follows the
+    # same table) inside the second pass
+    ev_uns = dep.event.copy()
+    i = 16384 + 1000
+    ev_uns[i], ev_uns[i + 3] = ev_uns[i + 3], ev_uns[i]
+    dep_uns = E.Deposits(dep.xyzt, dep.nph, ev_uns, dep.e_kev, nev)
+    ref_uns = _oracle_maps(K.C2_BOX, K.C2_OP, RO8, dep_uns, 7, K.C2_TAU)
+    darr_uns = (darr[0], darr[1], torch.from_numpy(ev_uns).to(dev))
+    ctx.set_event_sessions(True, 1024)
+    try:
+        E.transport_events(K.C2_BOX, K.C2_OP, RO8, dep_uns, seed=7, tau_ns=K.C2_TAU, ctx=ctx,
+                           device_arrays=darr_uns, out=out)
+    finally:
+        ctx.set_event_sessions(False, 1024)
+    assert np.array_equal(out[0].cpu().numpy().reshape(nev, -1), ref_uns["maps"])
+    assert np.array_equal(out[2].cpu().numpy(), ref_uns["npe"])
+
+
 # ---- events owned by workgroups ---------------------------------------------------
 # With deposits ordered by event, workgroups own whole events and write each map row once
 # (helpers' partial maps merged afterwards); otherwise, or when switched off, every
