@@ -45,8 +45,11 @@ def workload():
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the first launches of a process run slower while the clocks come up (a fresh
+    # process's launches went 6.3, 6.1, 5.9, 5.7, 5.6 ms before settling): ten untimed steps
+    # reach the steady state, and fifty timed steps take about 0.3 s
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--events", type=int, default=10_000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
