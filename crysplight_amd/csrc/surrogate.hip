@@ -316,16 +316,19 @@ __global__ void pack_inputs(const float *X, int M, int Mpad, int n_in, int kin_p
 
 // ------------------------------------------------------------ the training step ----
 // A step is four launches: mlp_rows (inputs, forward, loss, the backward data path and the
-// loss partials, one workgroup per 64 rows), dw_grouped (the weight gradients of every layer
+// loss partials, one workgroup per 64 or 128 rows), dw_grouped (the weight gradients of every layer
 // as split-K products, with the bias row sums), reduce_grads (the partials summed in a fixed
 // order) and adam_refresh (Adam, the bf16 weight copies).  Every sum runs in a fixed order, so a
 // training run is reproducible.
 
 constexpr int AMAX = 256;                           // widest layer (hidden, n_out, kin_pad)
 constexpr int CBLK = AMAX / 32;                     // 32-column blocks of the widest layer
-constexpr int RB = 64;                              // rows per mlp_rows workgroup: two groups of 32
-constexpr int RT = 64 * CBLK * (RB / 32);           // its threads: 16 waves, wave w on column
-                                                    // block w % CBLK of row group w / CBLK
+// mlp_rows: NB 32-row blocks per wave (1 or 2), so 64 NB rows per workgroup in two row
+// groups; 16 waves, wave w on column block w % CBLK of row group w / CBLK.  Two blocks per
+// wave halve the weight staging and barriers per row; one block gives twice the workgroups,
+// for batches too small to fill the chip with the larger ones.
+constexpr int RB_MIN = 64, RB_MAX = 128;            // rows per workgroup: NB = 1, 2
+constexpr int RT = 64 * CBLK * 2;                   // threads per workgroup
 constexpr int KB = 128;                             // K staged in LDS per phase (mlp_rows)
 
 struct RowArgs {
@@ -408,30 +411,26 @@ struct BStream {
     }
 };
 
-// out[RB x N] = A[RB x K] * B^T for the next GEMM of the stream (phases q .. q + np - 1),
-// A resident in LDS.  Wave w owns the 32 x 32 block of column block w % CBLK and row group
-// w / CBLK, lane element i (0..15) of it at (row, col); the two row groups share every
-// staged slice of B.  pre(i, row, col, u, v, r) loads what the epilogue of an element needs into
-// u, v, r before the K loop, so those loads complete under the MFMAs;
-// epi(i, row, col, acc, u, v, r) stores the element.
-template <class Pre, class Epi>
+// out[64 NB x N] = A[64 NB x K] * B^T (+ bias) for the next GEMM of the stream (phases
+// q .. q + np - 1), A resident in LDS.  Wave w owns column block w % CBLK of the rows of
+// row group w / CBLK: NB 32 x 32 blocks, which share every B fragment it reads (NB MFMAs
+// per fragment), and all 16 waves share every staged slice of B.  Lane element i
+// (0 .. 16 NB - 1) of the wave's blocks sits at (row, col); epi(i, row, col, acc + bias[col])
+// stores it.  The bias (null: none) is read before the K loop, so the load completes under
+// the MFMAs.  The K loop ends with a barrier, so the epilogue may write over A.
+template <int NB, class Epi>
 __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
                                           BStream &bs, int &q, __bf16 (*sB)[KB + LDS_PAD],
-                                          Pre pre, Epi epi)
+                                          const float *bias, Epi epi)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int cb = wave % CBLK, rbase = (wave / CBLK) * 32;
+    const int cb = wave % CBLK, rbase = (wave / CBLK) * 32 * NB;
     const bool mine = cb < N / 32;
     const int col = cb * 32 + (lane & 31);
-    float pu[16], pv[16], pr[16];
-    if (mine) {
+    const float u = mine && bias ? bias[col] : 0.0f;
+    f32x16 acc0, acc1;                              // acc1: the second block (NB = 2)
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            pre(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, pu[i], pv[i], pr[i]);
-    }
-    f32x16 acc;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < 16; ++i) { acc0[i] = 0.0f; acc1[i] = 0.0f; }
     for (int k0 = 0; k0 < K; k0 += KB, ++q) {
         const int kp = min(KB, K - k0);             // a multiple of 64
         bs.stash(N, kp, sB);
@@ -440,9 +439,13 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
         if (mine) {
             for (int kk = 0; kk < kp; kk += 16) {
                 const int r = lane & 31, kq = kk + 8 * (lane >> 5);
-                const bf16x8 a = *(const bf16x8 *)&A[rbase + r][k0 + kq];
-                const bf16x8 b = *(const bf16x8 *)&sB[cb * 32 + r][kq];
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                const bf16x8 bv = *(const bf16x8 *)&sB[cb * 32 + r][kq];
+                const bf16x8 a0 = *(const bf16x8 *)&A[rbase + r][k0 + kq];
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc0, 0, 0, 0);
+                if constexpr (NB == 2) {
+                    const bf16x8 a1 = *(const bf16x8 *)&A[rbase + 32 + r][k0 + kq];
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc1, 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -450,17 +453,29 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     if (mine) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-            epi(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i], pu[i], pv[i], pr[i]);
+            epi(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc0[i] + u);
+        if constexpr (NB == 2) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                epi(16 + i, rbase + 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc1[i] + u);
+        }
     }
 }
 
 // The RB rows of act (bf16, N columns) into dst transposed: dst[c * ld + r0 + r].  A
-// thread gathers 8 rows of a column from LDS and stores them as one 16-byte chunk.
+// thread gathers 8 rows of a column from LDS and stores them as one 16-byte chunk.  The
+// lanes of a wave take 64 consecutive columns of one 8-row chunk (N is a multiple of 64):
+// their 2-byte reads of a row fall in 32 distinct banks.  With the lanes along the rows
+// instead (16 lanes storing a column's 128 rows as one run), every read of a wave hit one
+// bank: rows 8 apart are 1056 dwords apart, a multiple of the 32 banks.  That cost 35 us
+// of a 330 us training step; an XOR swizzle of the row layout avoided the conflicts with
+// coalesced runs, ran no faster, and its address arithmetic pushed the kernel into spills.
+template <int RB>
 __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD], int N, __bf16 *dst,
                                              int64_t ld, int r0)
 {
     for (int c = threadIdx.x; c < N * (RB / 8); c += RT) {
-        const int col = c / (RB / 8), r = (c % (RB / 8)) * 8;
+        const int col = c % N, r = (c / N) * 8;
         bf16x8 v;
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = act[r + j][col];
@@ -468,22 +483,25 @@ __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD]
     }
 }
 
-// One workgroup per RB rows of the batch:
+// One workgroup per RB = 64 NB rows of the batch:
 //   pack (x - shift) * scale to bf16;
-//   forward: hidden layers Z = A W^T + b, H = gelu(Z) (bf16) and gelu'(Z) (fp32, for the
+//   forward: hidden layers Z = A W^T + b, H = gelu(Z) (bf16) and gelu'(Z) (bf16, for the
 //   backward);
 //   loss of the last layer's output y against t with weight w, normalized by 1 / sum w:
 //     columns c <  n_pois: Poisson NLL of a log-rate, w (e^y - t y), gradient w (e^y - t);
 //     columns c >= n_pois: weighted squared error w (y - t)^2, gradient 2 w (y - t);
 //   backward data path: dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1});
 //   the loss partial (the bias gradients come from dw_grouped, which reads every dZ_l).
+// Each layer's rows overwrite the previous layer's in one LDS buffer (a GEMM's K loop has
+// ended, with a barrier, before its epilogue writes).
 // Padded rows (>= M) carry zero inputs and a zero gradient.
 // TRAIN = false is the inference pass: the forward layers only, the last one's output (with
 // its bias) stored to a.Y, nothing else written.
-template <int L, bool TRAIN = true>
+template <int L, bool TRAIN, int NB>
 __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 {
-    __shared__ __bf16 act[2][RB][AMAX + LDS_PAD];
+    constexpr int RB = 64 * NB;
+    __shared__ __bf16 act[RB][AMAX + LDS_PAD];
     __shared__ __bf16 sB[AMAX][KB + LDS_PAD];
     __shared__ float red[RT / 64];
     const int tid = threadIdx.x;
@@ -501,99 +519,84 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         const int r = i / a.kin_pad, c = i % a.kin_pad;
         float v = 0.0f;
         if (r0 + r < a.M && c < a.n_in) v = (Xraw[(size_t)(r0 + r) * a.n_in + c] - a.shift[c]) * a.scale[c];
-        act[0][r][c] = to_bf16(v);
+        act[r][c] = to_bf16(v);
     }
     __syncthreads();
-    if (TRAIN) rows_store_T(act[0], a.kin_pad, a.XT, a.Mpad, r0);
+    if (TRAIN) rows_store_T<RB>(act, a.kin_pad, a.XT, a.Mpad, r0);
 
-    int cur = 0;
     float lsum = 0.0f;
     // gelu'(Z) of the hidden layers, kept in registers: the backward product of layer l + 1
     // has the forward layer l's shape, and every element stays with the lane that made it
-    // (as bf16, two to a register: 24 registers for three hidden layers instead of 48)
-    uint32_t gpr[L - 1][8];
+    // (as bf16, two to a register: 8 NB registers per hidden layer for the wave's 16 NB
+    // elements)
+    uint32_t gpr[L - 1][8 * NB];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         const int N = a.dout[l];
-        __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
-        const float *bias = a.bias[l];
         if (l + 1 < L) {
-            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int row, int col, float &u, float &, float &) { u = bias[col]; },
-                      [&](int i, int row, int col, float v, float u, float, float) {
-                          v += u;
-                          if (TRAIN) {
-                              const uint32_t hb = bf16_bits(gelu_grad(v));
-                              uint32_t &w = gpr[l < L - 1 ? l : 0][i >> 1];
-                              w = (i & 1) ? (w | (hb << 16)) : hb;
-                          }
-                          out[row][col] = to_bf16(gelu(v));
-                      });
+            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int i, int row, int col, float v) {
+                if (TRAIN) {
+                    const uint32_t hb = bf16_bits(gelu_grad(v));
+                    uint32_t &w = gpr[l < L - 1 ? l : 0][i >> 1];
+                    w = (i & 1) ? (w | (hb << 16)) : hb;
+                }
+                act[row][col] = to_bf16(gelu(v));
+            });
         } else if (!TRAIN) {
             const int M = a.M;
             float *Y = a.Y;
-            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int row, int col, float &u, float &, float &) { u = bias[col]; },
-                      [&](int, int row, int col, float y, float u, float, float) {
-                          const int rg = r0 + row;
-                          if (rg < M) Y[(size_t)rg * N + col] = y + u;
-                      });
+            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int, int row, int col, float y) {
+                const int rg = r0 + row;
+                if (rg < M) Y[(size_t)rg * N + col] = y;
+            });
         } else {
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
-            rows_gemm(act[cur], a.dpad[l], N, bs, q, sB,
-                      [&](int, int, int, float &, float &, float &) {},
-                      [&](int, int row, int col, float y, float, float, float) {
-                          // targets, weights and rate offsets are read here (four waves per
-                          // SIMD hide the latency; prefetched, they would not fit in registers)
-                          y += bias[col];
-                          const int rg = r0 + row;
-                          float g = 0.0f;
-                          if (rg < M) {
-                              const size_t o = (size_t)rg * N + col;
-                              const float t = T[o];
-                              const float w = W ? W[o] : 1.0f;
-                              const float r = (R0 && col < n_pois) ? R0[(size_t)rg * n_pois + col] : 0.0f;
-                              if (col < n_pois) {
-                                  // rate = e^y + r (r: a known part of the rate, 0 without
-                                  // offsets): w (rate - t log rate), gradient w (1 - t/rate) e^y
-                                  const float e = __expf(y);
-                                  if (R0) {
-                                      const float rate = e + r;
-                                      lsum += w * (rate - t * __logf(rate)) * inv_wsum;
-                                      g = w * (e - t * (e / rate)) * inv_wsum;
-                                  } else {
-                                      lsum += w * (e - t * y) * inv_wsum;
-                                      g = w * (e - t) * inv_wsum;
-                                  }
-                              } else {
-                                  const float d = y - t;
-                                  lsum += w * d * d * inv_wsum;
-                                  g = 2.0f * w * d * inv_wsum;
-                              }
-                          }
-                          out[row][col] = to_bf16(g);
-                      });
+            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int, int row, int col, float y) {
+                // targets, weights and rate offsets are read here (four waves per SIMD hide
+                // the latency; prefetched, they would not fit in registers)
+                const int rg = r0 + row;
+                float g = 0.0f;
+                if (rg < M) {
+                    const size_t o = (size_t)rg * N + col;
+                    const float t = T[o];
+                    const float w = W ? W[o] : 1.0f;
+                    const float r = (R0 && col < n_pois) ? R0[(size_t)rg * n_pois + col] : 0.0f;
+                    if (col < n_pois) {
+                        // rate = e^y + r (r: a known part of the rate, 0 without offsets):
+                        // w (rate - t log rate), gradient w (1 - t/rate) e^y
+                        const float e = __expf(y);
+                        if (R0) {
+                            const float rate = e + r;
+                            lsum += w * (rate - t * __logf(rate)) * inv_wsum;
+                            g = w * (e - t * (e / rate)) * inv_wsum;
+                        } else {
+                            lsum += w * (e - t * y) * inv_wsum;
+                            g = w * (e - t) * inv_wsum;
+                        }
+                    } else {
+                        const float d = y - t;
+                        lsum += w * d * d * inv_wsum;
+                        g = 2.0f * w * d * inv_wsum;
+                    }
+                }
+                act[row][col] = to_bf16(g);
+            });
         }
         __syncthreads();
-        cur ^= 1;
-        if (TRAIN) rows_store_T(act[cur], N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
+        if (TRAIN) rows_store_T<RB>(act, N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
     }
     if (!TRAIN) return;
 
 #pragma unroll
     for (int l = L - 1; l >= 1; --l) {
         const int N = a.din[l];                     // = dout[l - 1]
-        __bf16 (*out)[AMAX + LDS_PAD] = act[cur ^ 1];
-        rows_gemm(act[cur], a.dout[l], N, bs, q, sB,
-                  [&](int, int, int, float &, float &, float &) {},
-                  [&](int i, int row, int col, float v, float, float, float) {
-                      const uint32_t w = gpr[l - 1][i >> 1];
-                      out[row][col] = to_bf16(v * __uint_as_float((i & 1) ? (w & 0xFFFF0000u) : (w << 16)));
-                  });
+        rows_gemm<NB>(act, a.dout[l], N, bs, q, sB, nullptr, [&](int i, int row, int col, float v) {
+            const uint32_t w = gpr[l - 1][i >> 1];
+            act[row][col] = to_bf16(v * __uint_as_float((i & 1) ? (w & 0xFFFF0000u) : (w << 16)));
+        });
         __syncthreads();
-        cur ^= 1;
-        rows_store_T(act[cur], N, a.dZT[l - 1], a.Mpad, r0);
+        rows_store_T<RB>(act, N, a.dZT[l - 1], a.Mpad, r0);
     }
 
     // loss partial of the workgroup, in a fixed order
@@ -676,12 +679,21 @@ __global__ void __launch_bounds__(256) reduce_grads(const RedArgs a)
         const int o = (int)(bi - a.bcum[l]);
         a.g[a.boff[l] + o] = sum_strided(a.db[l] + o, a.ks[l], a.dout[l]);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const float s = sum_strided(a.lpart, a.n_wg, 1);
-        a.loss[a.ctr && a.loss_by_batch ? a.ctr[0] : 0] = s;
-        if (a.ctr) {
-            a.ctr[0] += 1;
-            a.ctr[1] += 1;
+    // the loss: lane t of the first wave adds partials t, t + 64, ... in order, then a fixed
+    // butterfly over the lanes (one thread alone spent a load round trip per 16 partials)
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        float s = 0.0f;
+#pragma unroll 8
+        for (int u = lane; u < a.n_wg; u += 64) s += a.lpart[u];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        if (lane == 0) {
+            a.loss[a.ctr && a.loss_by_batch ? a.ctr[0] : 0] = s;
+            if (a.ctr) {
+                a.ctr[0] += 1;
+                a.ctr[1] += 1;
+            }
         }
     }
 }
@@ -1158,6 +1170,7 @@ static int sfail(int code, const std::string &m) { s_err = m; return code; }
 
 struct crysp_mlp {
     int device = 0;
+    int n_cu = 256;                                 // compute units of the device
     hipStream_t stream = nullptr;
     int n_in = 4, hidden = 256, n_out = 128, n_layers = 4, kin_pad = 64;
     std::vector<int> din, dout, dpad;               // per layer: in, out, padded in
@@ -1229,7 +1242,7 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
     SCHK(hipMalloc(&n->Y, (size_t)cap * n->n_out * 4));
     SCHK(hipMalloc(&n->dY, (size_t)cap * n->n_out * 2));
     size_t part = 0, bpart = 0;
-    const int n_wg = (cap + RB - 1) / RB;
+    const int n_wg = (cap + RB_MIN - 1) / RB_MIN;
     n->part_off.assign(n->n_layers, 0);
     n->bpart_off.assign(n->n_layers, 0);
     for (int l = 0; l < n->n_layers; ++l) {
@@ -1259,16 +1272,48 @@ static void gemm(hipStream_t s, const GemmArgs &a, int k_slices = 1)
 }
 
 
+// mlp_rows for L layers with NB 32-row blocks per wave (64 NB rows per workgroup)
+template <bool TRAIN, int NB>
+static void launch_rows_nb(int L, int n_wg, hipStream_t s, const RowArgs &ra)
+{
+    const dim3 grid((unsigned)n_wg), block(RT);
+    switch (L) {                                    // the layer count is a template argument, so
+    case 2: hipLaunchKernelGGL((mlp_rows<2, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 3: hipLaunchKernelGGL((mlp_rows<3, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 4: hipLaunchKernelGGL((mlp_rows<4, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 5: hipLaunchKernelGGL((mlp_rows<5, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 6: hipLaunchKernelGGL((mlp_rows<6, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 7: hipLaunchKernelGGL((mlp_rows<7, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 8: hipLaunchKernelGGL((mlp_rows<8, TRAIN, NB>), grid, block, 0, s, ra); break;
+    default: hipLaunchKernelGGL((mlp_rows<9, TRAIN, NB>), grid, block, 0, s, ra); break;
+    }                                               // the layer loops unroll (gelu' in registers)
+}
+
+// Row blocks per wave for a batch of `rows`: two (128-row workgroups) once they give every
+// compute unit at least two workgroups, else one (64-row workgroups, twice as many).
+static int rows_nb(const crysp_mlp *n, int64_t rows)
+{
+    return rows >= (int64_t)2 * n->n_cu * RB_MAX ? 2 : 1;
+}
+
+// mlp_rows over Mpad rows (a multiple of 64 NB)
+template <bool TRAIN>
+static void launch_rows(int L, int nb, int Mpad, hipStream_t s, const RowArgs &ra)
+{
+    if (nb == 2) launch_rows_nb<TRAIN, 2>(L, Mpad / RB_MAX, s, ra);
+    else launch_rows_nb<TRAIN, 1>(L, Mpad / RB_MIN, s, ra);
+}
+
 // Inference on M rows of device inputs into device outputs Y [M][n_out]: one launch of the
-// row-block kernel's forward pass (RB rows per workgroup; activations stay in LDS, weights
-// stream through it).
+// row-block kernel's forward pass (activations stay in LDS, weights stream through it).
 static int mlp_infer_dev(crysp_mlp *n, const float *Xdev, int64_t M, float *Ydev)
 {
     const int L = n->n_layers;
     if (L > MAXL) return sfail(CRYSP_EINVAL, "too many layers");
-    if (M > ((int64_t)1 << 31) - RB) return sfail(CRYSP_ERANGE, "more than 2^31 rows");
+    if (M > ((int64_t)1 << 31) - RB_MAX) return sfail(CRYSP_ERANGE, "more than 2^31 rows");
+    const int nb = rows_nb(n, M), rb = 64 * nb;
     RowArgs ra{};
-    ra.L = L; ra.M = (int)M; ra.Mpad = (int)(((M + RB - 1) / RB) * RB); ra.n_in = n->n_in;
+    ra.L = L; ra.M = (int)M; ra.Mpad = (int)(((M + rb - 1) / rb) * rb); ra.n_in = n->n_in;
     ra.kin_pad = n->kin_pad; ra.n_out = n->n_out; ra.n_pois = n->n_pois;
     for (int l = 0; l < L; ++l) {
         ra.din[l] = n->din[l]; ra.dpad[l] = n->dpad[l]; ra.dout[l] = n->dout[l];
@@ -1276,17 +1321,7 @@ static int mlp_infer_dev(crysp_mlp *n, const float *Xdev, int64_t M, float *Ydev
     }
     ra.Xraw = Xdev; ra.shift = n->shift; ra.scale = n->scale;
     ra.Y = Ydev;
-    const dim3 grid((unsigned)(ra.Mpad / RB));
-    switch (L) {
-    case 2: hipLaunchKernelGGL((mlp_rows<2, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 3: hipLaunchKernelGGL((mlp_rows<3, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 4: hipLaunchKernelGGL((mlp_rows<4, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 5: hipLaunchKernelGGL((mlp_rows<5, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 6: hipLaunchKernelGGL((mlp_rows<6, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 7: hipLaunchKernelGGL((mlp_rows<7, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    case 8: hipLaunchKernelGGL((mlp_rows<8, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    default: hipLaunchKernelGGL((mlp_rows<9, false>), grid, dim3(RT), 0, n->stream, ra); break;
-    }
+    launch_rows<false>(L, nb, ra.Mpad, n->stream, ra);
     SCHK(hipGetLastError());
     return CRYSP_OK;
 }
@@ -1300,8 +1335,9 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
                         bool loss_by_batch, int64_t *ctr)
 {
     const int L = n->n_layers;
-    if (L > MAXL || Mpad % RB || Mpad > n->cap) return sfail(CRYSP_EINVAL, "bad training shape");
-    const int n_wg = Mpad / RB;
+    if (L > MAXL || Mpad % RB_MAX || Mpad > n->cap) return sfail(CRYSP_EINVAL, "bad training shape");
+    const int nb = rows_nb(n, Mpad);
+    const int n_wg = Mpad / (64 * nb);
     RowArgs ra{};
     ra.L = L; ra.M = M; ra.Mpad = Mpad; ra.n_in = n->n_in; ra.kin_pad = n->kin_pad; ra.n_out = n->n_out;
     ra.n_pois = n->n_pois;
@@ -1315,16 +1351,7 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     ra.Xraw = Xraw; ra.T = T; ra.W = W; ra.R0 = R0; ra.shift = n->shift; ra.scale = n->scale;
     ra.inv_wsum = (float)(1.0 / wsum);
     ra.bidx = ctr;
-    switch (L) {                                    // the layer count is a template argument, so
-    case 2: hipLaunchKernelGGL(mlp_rows<2>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 3: hipLaunchKernelGGL(mlp_rows<3>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 4: hipLaunchKernelGGL(mlp_rows<4>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 5: hipLaunchKernelGGL(mlp_rows<5>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 6: hipLaunchKernelGGL(mlp_rows<6>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 7: hipLaunchKernelGGL(mlp_rows<7>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    case 8: hipLaunchKernelGGL(mlp_rows<8>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    default: hipLaunchKernelGGL(mlp_rows<9>, dim3(n_wg), dim3(RT), 0, n->stream, ra); break;
-    }                                               // the layer loops unroll (gelu' in registers)
+    launch_rows<true>(L, nb, Mpad, n->stream, ra);
 
     // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad], both operands stored
     // transposed (k = the batch row contiguous), every layer in one launch, as the partial
@@ -1436,6 +1463,9 @@ crysp_mlp *crysp_mlp_create(int device, int32_t n_in, int32_t hidden, int32_t n_
     if (hipSetDevice(device) != hipSuccess) { sfail(CRYSP_EHIP, "hipSetDevice"); return nullptr; }
     auto *n = new crysp_mlp();
     n->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        n->n_cu = cus;
     n->n_in = n_in; n->hidden = hidden; n->n_out = n_out;
     n->n_layers = n_hidden_layers + 1;
     n->kin_pad = 64;
@@ -1598,7 +1628,7 @@ int crysp_mlp_gradient(crysp_mlp *n, const float *X, const float *T, const float
     if (!n || !X || !T || M < 1) return sfail(CRYSP_EINVAL, "bad arguments");
     if (R0 && n->n_pois < 1) return sfail(CRYSP_EINVAL, "rate offsets need Poisson outputs");
     SCHK(hipSetDevice(n->device));
-    const int mpad = ((M + 63) / 64) * 64;
+    const int mpad = ((M + RB_MAX - 1) / RB_MAX) * RB_MAX;     // whole row-kernel workgroups
     int rc = mlp_alloc_batch(n, mpad);
     if (rc) return rc;
     Staged sx, st, sw, sr;
@@ -1645,7 +1675,8 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
         return sfail(CRYSP_EINVAL, "training arrays must be device-resident");
     if (R0 && n->n_pois < 1) return sfail(CRYSP_EINVAL, "rate offsets need Poisson outputs");
     SCHK(hipSetDevice(n->device));
-    int rc = mlp_alloc_batch(n, batch);
+    const int mpad = ((batch + RB_MAX - 1) / RB_MAX) * RB_MAX; // whole row-kernel workgroups
+    int rc = mlp_alloc_batch(n, mpad);
     if (rc) return rc;
     const double wsum = (double)batch * n->n_out * (W ? mean_row_weight : 1.0f);
     if (losses && n_batches > n->dloss_cap) {
@@ -1667,7 +1698,7 @@ int crysp_mlp_train_batches(crysp_mlp *n, const float *X, const float *T, const 
     auto enqueue_step = [&](const float *x, const float *t, const float *w, const float *r0,
                             float *lout, int64_t *ctr, const float *dlr, float c1, float c2) -> int {
         int r;
-        if ((r = mlp_grad_dev(n, x, t, w, r0, batch, batch, wsum, lout ? lout : n->loss,
+        if ((r = mlp_grad_dev(n, x, t, w, r0, batch, mpad, wsum, lout ? lout : n->loss,
                               lout != nullptr, ctr)))
             return r;
         mlp_adam(n, lr, beta1, beta2, eps, c1, c2, weight_decay, ctr ? ctr + 1 : nullptr, dlr);
