@@ -174,12 +174,15 @@ def test_gemm_layout_against_numpy(a_t, b_t):
     (256, 3, 128, 4096),       # 32 slices of two stages
     (192, 2, 64, 1088),        # 64-row weight-gradient tiles; one slice of 17 stages
     (64, 1, 192, 200),         # padded rows (200 of 256), 64-wide hidden layer
+    (256, 3, 128, 65600),      # 128-row workgroups (two row blocks per wave), the last padded
 ])
 def test_mlp_gradient_against_fp64_oracle(hidden, n_hidden, n_out, rows):
     """Forward outputs, loss and every layer's weight and bias gradient of the device step
     against the FP64 oracle on the same bf16-rounded weights and inputs, over network shapes
     and batch sizes that take each tile shape of the weight-gradient kernel and several
-    batch-slice layouts (the bias gradients are the slices' dZ row sums there)."""
+    batch-slice layouts (the bias gradients are the slices' dZ row sums there), and both
+    row-kernel shapes (64-row workgroups, and 128-row ones once the batch gives every compute
+    unit two)."""
     from crysplight_amd.surrogate import MLP
     net = MLP(4, hidden, n_hidden, n_out, seed=3)
     rng = np.random.default_rng(2)
