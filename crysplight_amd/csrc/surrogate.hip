@@ -412,23 +412,31 @@ struct BStream {
 };
 
 // out[64 NB x N] = A[64 NB x K] * B^T (+ bias) for the next GEMM of the stream (phases
-// q .. q + np - 1), A resident in LDS.  Wave w owns column block w % CBLK of the rows of
-// row group w / CBLK: NB 32 x 32 blocks, which share every B fragment it reads (NB MFMAs
-// per fragment), and all 16 waves share every staged slice of B.  Lane element i
-// (0 .. 16 NB - 1) of the wave's blocks sits at (row, col); epi(i, row, col, acc + bias[col])
-// stores it.  The bias (null: none) is read before the K loop, so the load completes under
-// the MFMAs.  The K loop ends with a barrier, so the epilogue may write over A.
-template <int NB, class Epi>
+// q .. q + np - 1), A resident in LDS, all 16 waves sharing every staged slice of B.  Wave w
+// owns column block w % CBLK of the rows of row group w / CBLK: NB 32 x 32 blocks that share
+// every B fragment it reads.  A forward layer and its backward product have the same width,
+// so each element stays with the lane that made it (gelu' is kept in registers between the
+// two).  SPREAD (NB = 2, at most 4 column blocks; for the last layer, whose gelu' is not
+// kept): the 16 or fewer blocks go one to a wave instead of two to each of 8 waves, so all
+// 16 waves share the epilogue.  Lane element i (0 .. 16 NB - 1) of the wave's blocks sits at
+// (row, col); epi(i, row, col, acc + bias[col]) stores it.  The bias (null: none) is read
+// before the K loop, so the load completes under the MFMAs.  The K loop ends with a barrier,
+// so the epilogue may write over A.
+template <int NB, bool SPREAD, class Epi>
 __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int K, int N,
                                           BStream &bs, int &q, __bf16 (*sB)[KB + LDS_PAD],
                                           const float *bias, Epi epi)
 {
+    static_assert(!SPREAD || NB == 2, "one block per wave already");
+    constexpr bool two = NB == 2 && !SPREAD;        // a second block in acc1
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int cb = wave % CBLK, rbase = (wave / CBLK) * 32 * NB;
-    const bool mine = cb < N / 32;
+    const int ncb = N / 32;
+    const int cb = SPREAD ? wave % ncb : wave % CBLK;
+    const int rbase = SPREAD ? (wave / ncb) * 32 : (wave / CBLK) * 32 * NB;
+    const bool mine = SPREAD ? wave < 4 * ncb : cb < ncb;
     const int col = cb * 32 + (lane & 31);
     const float u = mine && bias ? bias[col] : 0.0f;
-    f32x16 acc0, acc1;                              // acc1: the second block (NB = 2)
+    f32x16 acc0, acc1;                              // acc1: the second block (two)
 #pragma unroll
     for (int i = 0; i < 16; ++i) { acc0[i] = 0.0f; acc1[i] = 0.0f; }
     for (int k0 = 0; k0 < K; k0 += KB, ++q) {
@@ -442,7 +450,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
                 const bf16x8 bv = *(const bf16x8 *)&sB[cb * 32 + r][kq];
                 const bf16x8 a0 = *(const bf16x8 *)&A[rbase + r][k0 + kq];
                 acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc0, 0, 0, 0);
-                if constexpr (NB == 2) {
+                if constexpr (two) {
                     const bf16x8 a1 = *(const bf16x8 *)&A[rbase + 32 + r][k0 + kq];
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc1, 0, 0, 0);
                 }
@@ -454,7 +462,7 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
 #pragma unroll
         for (int i = 0; i < 16; ++i)
             epi(i, rbase + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc0[i] + u);
-        if constexpr (NB == 2) {
+        if constexpr (two) {
 #pragma unroll
             for (int i = 0; i < 16; ++i)
                 epi(16 + i, rbase + 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc1[i] + u);
@@ -497,7 +505,7 @@ __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD]
 // Padded rows (>= M) carry zero inputs and a zero gradient.
 // TRAIN = false is the inference pass: the forward layers only, the last one's output (with
 // its bias) stored to a.Y, nothing else written.
-template <int L, bool TRAIN, int NB>
+template <int L, bool TRAIN, int NB, bool SPREAD_LAST>
 __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 {
     constexpr int RB = 64 * NB;
@@ -514,6 +522,13 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
     BStream bs(a, TRAIN);
     bs.load(0);
     int q = 0;
+    // the product act * B^T of the stream's next GEMM (the block mappings: rows_gemm)
+    auto gemm = [&](int K, int N, const float *bias, auto epi) {
+        rows_gemm<NB, false>(act, K, N, bs, q, sB, bias, epi);
+    };
+    auto gemm_last = [&](int K, int N, const float *bias, auto epi) {
+        rows_gemm<NB, SPREAD_LAST>(act, K, N, bs, q, sB, bias, epi);
+    };
 
     for (int i = tid; i < RB * a.kin_pad; i += RT) {
         const int r = i / a.kin_pad, c = i % a.kin_pad;
@@ -534,7 +549,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
     for (int l = 0; l < L; ++l) {
         const int N = a.dout[l];
         if (l + 1 < L) {
-            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int i, int row, int col, float v) {
+            gemm(a.dpad[l], N, a.bias[l], [&](int i, int row, int col, float v) {
                 if (TRAIN) {
                     const uint32_t hb = bf16_bits(gelu_grad(v));
                     uint32_t &w = gpr[l < L - 1 ? l : 0][i >> 1];
@@ -545,14 +560,14 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         } else if (!TRAIN) {
             const int M = a.M;
             float *Y = a.Y;
-            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int, int row, int col, float y) {
+            gemm_last(a.dpad[l], N, a.bias[l], [&](int, int row, int col, float y) {
                 const int rg = r0 + row;
                 if (rg < M) Y[(size_t)rg * N + col] = y;
             });
         } else {
             const int M = a.M, n_pois = a.n_pois;
             const float inv_wsum = a.inv_wsum;
-            rows_gemm<NB>(act, a.dpad[l], N, bs, q, sB, a.bias[l], [&](int, int row, int col, float y) {
+            gemm_last(a.dpad[l], N, a.bias[l], [&](int, int row, int col, float y) {
                 // targets, weights and rate offsets are read here (four waves per SIMD hide
                 // the latency; prefetched, they would not fit in registers)
                 const int rg = r0 + row;
@@ -591,7 +606,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
 #pragma unroll
     for (int l = L - 1; l >= 1; --l) {
         const int N = a.din[l];                     // = dout[l - 1]
-        rows_gemm<NB>(act, a.dout[l], N, bs, q, sB, nullptr, [&](int i, int row, int col, float v) {
+        gemm(a.dout[l], N, nullptr, [&](int i, int row, int col, float v) {
             const uint32_t w = gpr[l - 1][i >> 1];
             act[row][col] = to_bf16(v * __uint_as_float((i & 1) ? (w & 0xFFFF0000u) : (w << 16)));
         });
@@ -1273,19 +1288,19 @@ static void gemm(hipStream_t s, const GemmArgs &a, int k_slices = 1)
 
 
 // mlp_rows for L layers with NB 32-row blocks per wave (64 NB rows per workgroup)
-template <bool TRAIN, int NB>
+template <bool TRAIN, int NB, bool SL>
 static void launch_rows_nb(int L, int n_wg, hipStream_t s, const RowArgs &ra)
 {
     const dim3 grid((unsigned)n_wg), block(RT);
     switch (L) {                                    // the layer count is a template argument, so
-    case 2: hipLaunchKernelGGL((mlp_rows<2, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 3: hipLaunchKernelGGL((mlp_rows<3, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 4: hipLaunchKernelGGL((mlp_rows<4, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 5: hipLaunchKernelGGL((mlp_rows<5, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 6: hipLaunchKernelGGL((mlp_rows<6, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 7: hipLaunchKernelGGL((mlp_rows<7, TRAIN, NB>), grid, block, 0, s, ra); break;
-    case 8: hipLaunchKernelGGL((mlp_rows<8, TRAIN, NB>), grid, block, 0, s, ra); break;
-    default: hipLaunchKernelGGL((mlp_rows<9, TRAIN, NB>), grid, block, 0, s, ra); break;
+    case 2: hipLaunchKernelGGL((mlp_rows<2, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 3: hipLaunchKernelGGL((mlp_rows<3, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 4: hipLaunchKernelGGL((mlp_rows<4, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 5: hipLaunchKernelGGL((mlp_rows<5, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 6: hipLaunchKernelGGL((mlp_rows<6, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 7: hipLaunchKernelGGL((mlp_rows<7, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    case 8: hipLaunchKernelGGL((mlp_rows<8, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
+    default: hipLaunchKernelGGL((mlp_rows<9, TRAIN, NB, SL>), grid, block, 0, s, ra); break;
     }                                               // the layer loops unroll (gelu' in registers)
 }
 
@@ -1297,11 +1312,14 @@ static int rows_nb(const crysp_mlp *n, int64_t rows)
 }
 
 // mlp_rows over Mpad rows (a multiple of 64 NB)
+// (128-row workgroups spread the last layer one block per wave when it has at most 4
+// column blocks)
 template <bool TRAIN>
 static void launch_rows(int L, int nb, int Mpad, hipStream_t s, const RowArgs &ra)
 {
-    if (nb == 2) launch_rows_nb<TRAIN, 2>(L, Mpad / RB_MAX, s, ra);
-    else launch_rows_nb<TRAIN, 1>(L, Mpad / RB_MIN, s, ra);
+    if (nb == 2 && ra.n_out <= 128) launch_rows_nb<TRAIN, 2, true>(L, Mpad / RB_MAX, s, ra);
+    else if (nb == 2) launch_rows_nb<TRAIN, 2, false>(L, Mpad / RB_MAX, s, ra);
+    else launch_rows_nb<TRAIN, 1, false>(L, Mpad / RB_MIN, s, ra);
 }
 
 // Inference on M rows of device inputs into device outputs Y [M][n_out]: one launch of the
