@@ -288,14 +288,26 @@ __global__ void __launch_bounds__(256) dw_grouped(const DwArgs a)
 {
     __shared__ __bf16 sA[2][DW_M][DW_KP + LDS_PAD];
     __shared__ __bf16 sB[2][DW_MAXN][DW_KP + LDS_PAD];
+    // Workgroups go to the 8 XCDs in turn (linear id mod 8), and each XCD has its own L2.
+    // The tiles of one batch slice read the same rows of the stored activations and
+    // gradients, so they are placed on one XCD: XCD x takes slices x S/8 .. (x+1) S/8 - 1,
+    // tile by tile, and a tile's re-read of its operand rows can hit that XCD's L2.
+    int tid_all = blockIdx.x, slice = blockIdx.y;
+    const int T = gridDim.x, S = gridDim.y;
+    if (S % 8 == 0) {
+        const int id = blockIdx.x + blockIdx.y * T;
+        const int j = id / 8;
+        slice = (id % 8) * (S / 8) + j / T;
+        tid_all = j % T;
+    }
     int l = 0;
-    while (l + 1 < a.n && (int)blockIdx.x >= a.t0[l + 1]) ++l;
-    const int tile = blockIdx.x - a.t0[l];
+    while (l + 1 < a.n && tid_all >= a.t0[l + 1]) ++l;
+    const int tile = tid_all - a.t0[l];
     const bool wm = a.M[l] % 128 == 0, wn = a.N[l] % 128 == 0;
-    if (wm && wn) dw_tile<128, 128>(a, l, tile, blockIdx.y, sA, sB);
-    else if (wm) dw_tile<128, 64>(a, l, tile, blockIdx.y, sA, sB);
-    else if (wn) dw_tile<64, 128>(a, l, tile, blockIdx.y, sA, sB);
-    else dw_tile<64, 64>(a, l, tile, blockIdx.y, sA, sB);
+    if (wm && wn) dw_tile<128, 128>(a, l, tile, slice, sA, sB);
+    else if (wm) dw_tile<128, 64>(a, l, tile, slice, sA, sB);
+    else if (wn) dw_tile<64, 128>(a, l, tile, slice, sA, sB);
+    else dw_tile<64, 64>(a, l, tile, slice, sA, sB);
 }
 
 // ---------------------------------------------------------------- small kernels ----
