@@ -306,6 +306,37 @@ def test_surrogate_end_to_end_small():
         r["lines"]["any_face_1mm"]["fallback_deposit_fraction"]
 
 
+@gpu
+def test_surrogate_events_three_thickness_mix():
+    """The configuration's event mix: gamma events over three crystal thicknesses in one
+    several-crystal queue, each crystal's deposits sampled from its own small surrogate with
+    MC fallback near the faces, against the plain MC of the same deposits. Every thickness
+    gets events, its energy sum stays within a few percent, and the position-binned map error
+    is reported for all of them together."""
+    from crysplight_amd import surrogate_eval as SE
+    from crysplight_amd.events import GammaMaterial
+    from crysplight_amd.params import Box
+    from tests import cases as K
+    sur = []
+    for t in (10.0, 20.0, 30.0):
+        s_, _ = SE.train_for_box(Box((25.6, 25.6, t)), K.C1_OP, K.C1_RO, tau_ns=1000.0,
+                                 n_samples=50_000, photons=4000, epochs=10, batch=4096)
+        sur.append(s_)
+    csi = GammaMaterial("xcom_CSI.csv", 4.51)
+    dep, part_of, _, boxes = SE.gamma_mix((10.0, 20.0, 30.0), 25.6, csi, 54_000.0, 3000, seed=5)
+    assert dep.n_events == 3000 and sorted(set(part_of.tolist())) == [0, 1, 2]
+    r = SE.compare_events(sur, boxes, K.C1_OP, K.C1_RO, dep, part_of, yield_per_mev=54_000.0,
+                          tau_ns=1000.0, rules={"any_face_1mm": {"any_face_mm": 1.0},
+                                                "corners_0.5mm": {"corner_mm": 0.5}})
+    for name, line in r["lines"].items():
+        assert set(line["by_thickness"]) == {"10mm", "20mm", "30mm"}, (name, line)
+        for t, v in line["by_thickness"].items():
+            assert abs(v["energy_sum_bias"]) < 0.06, (name, t, v)
+        assert line["position_binned"]["bins"] > 0, (name, line)
+    assert r["lines"]["corners_0.5mm"]["fallback_deposit_fraction"] < \
+        r["lines"]["any_face_1mm"]["fallback_deposit_fraction"]
+
+
 def test_training_targets_fold_cells_and_weights():
     """Targets of a tiny training set: folded cell order (the map of a folded point),
     per-photon rates, standardized mean times (0 where a cell saw nothing) and weights."""
