@@ -169,8 +169,10 @@ __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g)
 constexpr int MAXL = 9;                             // layers of the MLP, at most
 
 // ------------------------------------------------------------- weight gradients ----
-// dW_l [out x in_pad] = dZ_l^T H_{l-1} over the batch: A = dZT [out][Mpad] and B = HT (or
-// the packed inputs XT) [in_pad][Mpad], both with the batch row contiguous.  The batch is
+// dW_l [out x in_pad] = dZ_l^T H_{l-1} over the batch: A = dZT (out wide) and B = HT (or
+// the packed inputs XT, in_pad wide), both in the chunked transposed layout mlp_rows stores:
+// element (batch row k, column m) of a width-D operand at ((k / 8) D + m) 8 + k % 8, so the
+// 8 batch rows an MFMA operand lane needs form one 16-byte chunk.  The batch is
 // split into ks slices of kslice rows (ks at most DW_KS whatever the batch, so the
 // reduction stays short); slice s's partial product goes to C + s * out * in_pad and
 // reduce_grads adds the slices in order.  A workgroup computes one DW_M x BN tile of one
@@ -190,7 +192,6 @@ struct DwArgs {
     float *db[MAXL];                               // bias partials [ks][M]
     int t0[MAXL + 1];                              // flat tile ranges of the layers
     int n;                                         // layers
-    int64_t ld;                                    // Mpad: row stride of A and B
     int kslice;                                    // batch rows per slice, a multiple of DW_KP
 };
 
@@ -206,18 +207,18 @@ __device__ __forceinline__ void dw_tile(const DwArgs &a, int l, int tile, int sl
     const int ntn = a.N[l] / BN;
     const int m0 = (tile / ntn) * BM_, n0 = (tile % ntn) * BN;
     const int wm = (wave >> 1) * (BM_ / 2), wn = (wave & 1) * (BN / 2);
-    const int64_t ld = a.ld;
+    const int64_t Ma = a.M[l], Nb = a.N[l];        // operand widths
     const int64_t kb = (int64_t)slice * a.kslice;
     const int kc = (tid & 7) * 8, r8 = tid >> 3;   // chunk: row r8 + 32 u, k kc .. kc + 7
-    const __bf16 *Ag = a.A[l] + (size_t)m0 * ld + kb + kc;
-    const __bf16 *Bg = a.B[l] + (size_t)n0 * ld + kb + kc;
+    const __bf16 *Ag = a.A[l] + ((kb + kc) / 8 * Ma + m0 + r8) * 8;
+    const __bf16 *Bg = a.B[l] + ((kb + kc) / 8 * Nb + n0 + r8) * 8;
     const bool rows = n0 == 0 && a.db[l];
     bf16x8 ra[CA], rb[CB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < CA; ++u) ra[u] = *(const bf16x8 *)(Ag + (size_t)(r8 + 32 * u) * ld + k0);
+        for (int u = 0; u < CA; ++u) ra[u] = *(const bf16x8 *)(Ag + k0 * Ma + 256 * u);
 #pragma unroll
-        for (int u = 0; u < CB; ++u) rb[u] = *(const bf16x8 *)(Bg + (size_t)(r8 + 32 * u) * ld + k0);
+        for (int u = 0; u < CB; ++u) rb[u] = *(const bf16x8 *)(Bg + k0 * Nb + 256 * u);
     };
     auto store = [&](int b) {
 #pragma unroll
@@ -349,9 +350,9 @@ struct RowArgs {
     const __bf16 *Wt[MAXL];                         // [out][in_pad]
     const __bf16 *Wn[MAXL];                         // [in][out]
     const float *bias[MAXL];
-    __bf16 *HT[MAXL];                               // hidden layers: activation, transposed [out][Mpad]
-    __bf16 *dZT[MAXL];                              // gradient of layer l's pre-activation, [out][Mpad]
-    __bf16 *XT;                                     // packed inputs, transposed [kin_pad][Mpad]
+    __bf16 *HT[MAXL];                               // hidden layers: activation, chunked transposed
+    __bf16 *dZT[MAXL];                              // gradient of layer l's pre-activation, the same
+    __bf16 *XT;                                     // packed inputs, the same (kin_pad wide)
     float *lpart;                                   // [n_wg] loss partials
     const float *Xraw, *T, *W, *shift, *scale;
     const float *R0;                                // rate offsets [M][n_pois] or null
@@ -482,24 +483,25 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
     }
 }
 
-// The RB rows of act (bf16, N columns) into dst transposed: dst[c * ld + r0 + r].  A
-// thread gathers 8 rows of a column from LDS and stores them as one 16-byte chunk.  The
-// lanes of a wave take 64 consecutive columns of one 8-row chunk (N is a multiple of 64):
-// their 2-byte reads of a row fall in 32 distinct banks.  With the lanes along the rows
-// instead (16 lanes storing a column's 128 rows as one run), every read of a wave hit one
-// bank: rows 8 apart are 1056 dwords apart, a multiple of the 32 banks.  That cost 35 us
-// of a 330 us training step; an XOR swizzle of the row layout avoided the conflicts with
-// coalesced runs, ran no faster, and its address arithmetic pushed the kernel into spills.
+// The RB rows of act (bf16, N columns) into dst in the chunked transposed layout of the
+// batch: element (row r, column c) at ((r / 8) N + c) 8 + r % 8, i.e. for each 8-row chunk
+// of the batch, the columns in order, each as the chunk's 8 rows (16 bytes).  A thread
+// gathers 8 rows of a column from LDS and stores them as one chunk; the lanes of a wave take
+// 64 consecutive columns of one 8-row chunk (N is a multiple of 64), so their 2-byte LDS
+// reads of a row fall in 32 distinct banks and their stores are one contiguous 1 KB run.
+// (With dst as [column][batch row] instead, a wave's 64 chunks went to 64 separate 16-byte
+// pieces of 64 lines, and lanes along the rows for contiguous runs put every LDS read of a
+// wave in one bank: rows 8 apart are 1056 dwords apart, a multiple of the 32 banks.)
 template <int RB>
-__device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD], int N, __bf16 *dst,
-                                             int64_t ld, int r0)
+__device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD], int N, __bf16 *dst, int r0)
 {
+    __bf16 *d = dst + (size_t)(r0 >> 3) * N * 8;
     for (int c = threadIdx.x; c < N * (RB / 8); c += RT) {
         const int col = c % N, r = (c / N) * 8;
         bf16x8 v;
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = act[r + j][col];
-        *(bf16x8 *)(dst + (size_t)col * ld + r0 + r) = v;
+        *(bf16x8 *)(d + (size_t)c * 8) = v;
     }
 }
 
@@ -549,7 +551,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         act[r][c] = to_bf16(v);
     }
     __syncthreads();
-    if (TRAIN) rows_store_T<RB>(act, a.kin_pad, a.XT, a.Mpad, r0);
+    if (TRAIN) rows_store_T<RB>(act, a.kin_pad, a.XT, r0);
 
     float lsum = 0.0f;
     // gelu'(Z) of the hidden layers, kept in registers: the backward product of layer l + 1
@@ -611,7 +613,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
             });
         }
         __syncthreads();
-        if (TRAIN) rows_store_T<RB>(act, N, l + 1 < L ? a.HT[l] : a.dZT[l], a.Mpad, r0);
+        if (TRAIN) rows_store_T<RB>(act, N, l + 1 < L ? a.HT[l] : a.dZT[l], r0);
     }
     if (!TRAIN) return;
 
@@ -623,7 +625,7 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
             act[row][col] = to_bf16(v * __uint_as_float((i & 1) ? (w & 0xFFFF0000u) : (w << 16)));
         });
         __syncthreads();
-        rows_store_T<RB>(act, N, a.dZT[l - 1], a.Mpad, r0);
+        rows_store_T<RB>(act, N, a.dZT[l - 1], r0);
     }
 
     // loss partial of the workgroup, in a fixed order
@@ -1384,11 +1386,11 @@ static int mlp_grad_dev(crysp_mlp *n, const float *Xraw, const float *T, const f
     launch_rows<true>(L, nb, Mpad, n->stream, ra);
 
     // dW_l[out][in] = dZ_l^T [out x Mpad] * Hin [Mpad x in_pad], both operands stored
-    // transposed (k = the batch row contiguous), every layer in one launch, as the partial
+    // chunked transposed (8 batch rows contiguous), every layer in one launch, as the partial
     // products of ks batch slices (with the bias row sums)
     const int ks = dw_slices(Mpad);
     DwArgs da{};
-    da.n = L; da.ld = Mpad; da.kslice = Mpad / ks;
+    da.n = L; da.kslice = Mpad / ks;
     RedArgs rd{};
     rd.L = L; rd.n_wg = n_wg;
     int tiles = 0;
