@@ -184,6 +184,9 @@ constexpr int MAXL = 9;                             // layers of the MLP, at mos
 // in a fixed order (each thread half a row per stage, in k order, stage after stage; the
 // two halves added last).
 constexpr int DW_M = 128, DW_KP = 64, DW_MAXN = 128, DW_KS = 32;
+// elements between consecutive 8-row groups of a chunked transposed operand of the given
+// width (padding these groups apart measured no faster)
+__host__ __device__ constexpr int64_t chunk_stride(int64_t width) { return width * 8; }
 
 struct DwArgs {
     const __bf16 *A[MAXL], *B[MAXL];               // dZT, HT (XT for layer 0)
@@ -209,22 +212,38 @@ __device__ __forceinline__ void dw_tile(const DwArgs &a, int l, int tile, int sl
     const int wm = (wave >> 1) * (BM_ / 2), wn = (wave & 1) * (BN / 2);
     const int64_t Ma = a.M[l], Nb = a.N[l];        // operand widths
     const int64_t kb = (int64_t)slice * a.kslice;
-    const int kc = (tid & 7) * 8, r8 = tid >> 3;   // chunk: row r8 + 32 u, k kc .. kc + 7
-    const __bf16 *Ag = a.A[l] + ((kb + kc) / 8 * Ma + m0 + r8) * 8;
-    const __bf16 *Bg = a.B[l] + ((kb + kc) / 8 * Nb + n0 + r8) * 8;
+    const int64_t sa = chunk_stride(Ma), sb = chunk_stride(Nb);
+    const __bf16 *Ag = a.A[l] + kb / 8 * sa + m0 * 8;
+    const __bf16 *Bg = a.B[l] + kb / 8 * sb + n0 * 8;
     const bool rows = n0 == 0 && a.db[l];
+    // chunk c = tid + 256 u of a stage of A: column m0 + c % BM_, batch rows k0 + 8 (c / BM_)
+    // onwards (B likewise with BN), so the 64 lanes of a wave read 64 consecutive chunks of
+    // one 8-row group, one contiguous 1 KB run, and write them to LDS rows 36 dwords apart:
+    // each 8 lanes of a 16-byte write cover the 32 banks once
     bf16x8 ra[CA], rb[CB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < CA; ++u) ra[u] = *(const bf16x8 *)(Ag + k0 * Ma + 256 * u);
+        for (int u = 0; u < CA; ++u) {
+            const int c = tid + 256 * u;
+            ra[u] = *(const bf16x8 *)(Ag + (k0 / 8 + c / BM_) * sa + (c % BM_) * 8);
+        }
 #pragma unroll
-        for (int u = 0; u < CB; ++u) rb[u] = *(const bf16x8 *)(Bg + k0 * Nb + 256 * u);
+        for (int u = 0; u < CB; ++u) {
+            const int c = tid + 256 * u;
+            rb[u] = *(const bf16x8 *)(Bg + (k0 / 8 + c / BN) * sb + (c % BN) * 8);
+        }
     };
     auto store = [&](int b) {
 #pragma unroll
-        for (int u = 0; u < CA; ++u) *(bf16x8 *)&sA[b][r8 + 32 * u][kc] = ra[u];
+        for (int u = 0; u < CA; ++u) {
+            const int c = tid + 256 * u;
+            *(bf16x8 *)&sA[b][c % BM_][(c / BM_) * 8] = ra[u];
+        }
 #pragma unroll
-        for (int u = 0; u < CB; ++u) *(bf16x8 *)&sB[b][r8 + 32 * u][kc] = rb[u];
+        for (int u = 0; u < CB; ++u) {
+            const int c = tid + 256 * u;
+            *(bf16x8 *)&sB[b][c % BN][(c / BN) * 8] = rb[u];
+        }
     };
     f32x16 acc[MT][NT];
 #pragma unroll
@@ -495,13 +514,14 @@ __device__ __forceinline__ void rows_gemm(const __bf16 (*A)[AMAX + LDS_PAD], int
 template <int RB>
 __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD], int N, __bf16 *dst, int r0)
 {
-    __bf16 *d = dst + (size_t)(r0 >> 3) * N * 8;
+    const int64_t sd = chunk_stride(N);
+    __bf16 *d = dst + (r0 >> 3) * sd;
     for (int c = threadIdx.x; c < N * (RB / 8); c += RT) {
         const int col = c % N, r = (c / N) * 8;
         bf16x8 v;
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = act[r + j][col];
-        *(bf16x8 *)(d + (size_t)c * 8) = v;
+        *(bf16x8 *)(d + (r >> 3) * sd + col * 8) = v;
     }
 }
 
@@ -583,7 +603,8 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
             const float inv_wsum = a.inv_wsum;
             gemm_last(a.dpad[l], N, a.bias[l], [&](int, int row, int col, float y) {
                 // targets, weights and rate offsets are read here (four waves per SIMD hide
-                // the latency; prefetched, they would not fit in registers)
+                // the latency; the targets alone loaded before the K loop measured no faster,
+                // and all three would not fit in registers)
                 const int rg = r0 + row;
                 float g = 0.0f;
                 if (rg < M) {
@@ -1267,17 +1288,18 @@ static int mlp_alloc_batch(crysp_mlp *n, int cap)
     n->Z.assign(n->n_layers, nullptr);
     n->H.assign(n->n_layers, nullptr);
     n->dZ.assign(n->n_layers, nullptr);
-    SCHK(hipMalloc(&n->X, (size_t)cap * n->kin_pad * 2));
+    const size_t groups = (size_t)(cap + 7) / 8;     // chunked transposed operands
+    SCHK(hipMalloc(&n->X, groups * chunk_stride(n->kin_pad) * 2));
     SCHK(hipMalloc(&n->Y, (size_t)cap * n->n_out * 4));
-    SCHK(hipMalloc(&n->dY, (size_t)cap * n->n_out * 2));
+    SCHK(hipMalloc(&n->dY, groups * chunk_stride(n->n_out) * 2));
     size_t part = 0, bpart = 0;
     const int n_wg = (cap + RB_MIN - 1) / RB_MIN;
     n->part_off.assign(n->n_layers, 0);
     n->bpart_off.assign(n->n_layers, 0);
     for (int l = 0; l < n->n_layers; ++l) {
         SCHK(hipMalloc(&n->Z[l], (size_t)cap * n->dout[l] * 4));
-        SCHK(hipMalloc(&n->H[l], (size_t)cap * n->dout[l] * 2));
-        SCHK(hipMalloc(&n->dZ[l], (size_t)cap * n->dout[l] * 2));
+        SCHK(hipMalloc(&n->H[l], groups * chunk_stride(n->dout[l]) * 2));
+        SCHK(hipMalloc(&n->dZ[l], groups * chunk_stride(n->dout[l]) * 2));
         // the slices' partial weight gradients and bias row sums (at most DW_KS slices)
         n->part_off[l] = part;
         part += (size_t)DW_KS * n->dout[l] * n->dpad[l];
