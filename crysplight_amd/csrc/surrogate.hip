@@ -516,12 +516,18 @@ __device__ __forceinline__ void rows_store_T(const __bf16 (*act)[AMAX + LDS_PAD]
 {
     const int64_t sd = chunk_stride(N);
     __bf16 *d = dst + (r0 >> 3) * sd;
-    for (int c = threadIdx.x; c < N * (RB / 8); c += RT) {
-        const int col = c % N, r = (c / N) * 8;
+    // chunk c = threadIdx.x + u RT is (8-row group c / N, column c % N): one division per
+    // thread, then steps of RT (RT / N groups and RT % N columns, with the carry)
+    const int dg = RT / N, dc = RT % N;
+    int g = threadIdx.x / N, col = threadIdx.x % N;
+    while (g < RB / 8) {
         bf16x8 v;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = act[r + j][col];
-        *(bf16x8 *)(d + (r >> 3) * sd + col * 8) = v;
+        for (int j = 0; j < 8; ++j) v[j] = act[8 * g + j][col];
+        *(bf16x8 *)(d + g * sd + col * 8) = v;
+        g += dg;
+        col += dc;
+        if (col >= N) { col -= N; ++g; }
     }
 }
 
@@ -564,11 +570,17 @@ __global__ void __launch_bounds__(RT) mlp_rows(const RowArgs a)
         rows_gemm<NB, SPREAD_LAST>(act, K, N, bs, q, sB, bias, epi);
     };
 
-    for (int i = tid; i < RB * a.kin_pad; i += RT) {
-        const int r = i / a.kin_pad, c = i % a.kin_pad;
-        float v = 0.0f;
-        if (r0 + r < a.M && c < a.n_in) v = (Xraw[(size_t)(r0 + r) * a.n_in + c] - a.shift[c]) * a.scale[c];
-        act[r][c] = to_bf16(v);
+    {   // element i = tid + u RT is (row i / kin_pad, column i % kin_pad), stepped as in rows_store_T
+        const int kp = a.kin_pad, dr = RT / kp, dc = RT % kp;
+        int r = tid / kp, c = tid % kp;
+        while (r < RB) {
+            float v = 0.0f;
+            if (r0 + r < a.M && c < a.n_in) v = (Xraw[(size_t)(r0 + r) * a.n_in + c] - a.shift[c]) * a.scale[c];
+            act[r][c] = to_bf16(v);
+            r += dr;
+            c += dc;
+            if (c >= kp) { c -= kp; ++r; }
+        }
     }
     __syncthreads();
     if (TRAIN) rows_store_T<RB>(act, a.kin_pad, a.XT, r0);
